@@ -1,0 +1,205 @@
+"""Seeded synthetic road-like graphs and coordinate-bisection partitions.
+
+Produces exactly the inputs the reference reads (README.md:25-66):
+  * arcs listed in both directions, 1-based in the Edge file ("a u v w"),
+  * the xxxID array (index, x, y, vertex id 0-based) and
+  * the xxxCut heap (2^H - 1 lines "h l r", [l, r) ranges into the ID array),
+so the same graph can be fed to the reference binary, the oracle and the GPU
+library.
+
+Generators:
+  grid_graph      -- rows x cols 4-neighbour grid, `drop` of the edges removed
+                     at random among non-spanning-tree edges (stays connected),
+                     weights round(scale * 1 * U[1, 1.5]).
+  delaunay_graph  -- uniform points in the unit square, Delaunay edges pruned to
+                     an average degree: the Euclidean spanning tree plus random
+                     extra Delaunay edges; weights round(scale * euclid * U[1,1.5]),
+                     at least 1.
+  bisect          -- recursive coordinate bisection (x at even depth, y at odd
+                     depth, split at the middle index) to height H.
+"""
+import os
+
+import numpy as np
+
+
+class Graph:
+    """n vertices, undirected simple edge list (eu < ev), int32 weights, coords."""
+
+    def __init__(self, n, eu, ev, ew, xy):
+        self.n = int(n)
+        self.eu = np.asarray(eu, np.int32)
+        self.ev = np.asarray(ev, np.int32)
+        self.ew = np.asarray(ew, np.int32)
+        self.xy = np.asarray(xy, np.float64)
+
+    @property
+    def m(self):
+        return len(self.eu)
+
+    def arcs(self):
+        """Both directions, grouped by source (the order the Edge file lists them)."""
+        u = np.concatenate([self.eu, self.ev])
+        v = np.concatenate([self.ev, self.eu])
+        w = np.concatenate([self.ew, self.ew])
+        order = np.lexsort((v, u))
+        return u[order].astype(np.int32), v[order].astype(np.int32), w[order].astype(np.int32)
+
+
+def _spanning_tree_mask(n, eu, ev, key):
+    """Kruskal on `key` (ascending); returns a bool mask of tree edges."""
+    parent = np.arange(n)
+
+    def find(a):
+        root = a
+        while parent[root] != root:
+            root = parent[root]
+        while parent[a] != root:
+            parent[a], a = root, parent[a]
+        return root
+
+    order = np.argsort(key, kind="stable")
+    mask = np.zeros(len(eu), bool)
+    cnt = 0
+    for e in order:
+        a, b = find(eu[e]), find(ev[e])
+        if a != b:
+            parent[a] = b
+            mask[e] = True
+            cnt += 1
+            if cnt == n - 1:
+                break
+    if cnt != n - 1:
+        raise ValueError("candidate edge set is disconnected")
+    return mask
+
+
+def _spanning_tree_scipy(n, eu, ev, key):
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+    # keys must be positive and distinct enough: add a tiny index tie-break
+    k = key.astype(np.float64) + 1e-9 * (np.arange(len(key)) + 1)
+    k = k - k.min() + 1.0
+    A = coo_matrix((k, (eu, ev)), shape=(n, n)).tocsr()
+    T = minimum_spanning_tree(A).tocoo()
+    tree = set(zip(np.minimum(T.row, T.col).tolist(), np.maximum(T.row, T.col).tolist()))
+    if len(tree) != n - 1:
+        raise ValueError("candidate edge set is disconnected")
+    a = np.minimum(eu, ev)
+    b = np.maximum(eu, ev)
+    return np.fromiter(((x, y) in tree for x, y in zip(a.tolist(), b.tolist())), bool, len(a))
+
+
+def grid_graph(rows=64, cols=64, drop=0.10, seed=1, scale=100):
+    rng = np.random.default_rng(seed)
+    idx = np.arange(rows * cols).reshape(rows, cols)
+    h = np.stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()], 1)
+    v = np.stack([idx[:-1, :].ravel(), idx[1:, :].ravel()], 1)
+    e = np.concatenate([h, v])
+    tree = _spanning_tree_mask(rows * cols, e[:, 0], e[:, 1], rng.random(len(e)))
+    n_drop = int(round(drop * len(e)))
+    cand = np.flatnonzero(~tree)
+    n_drop = min(n_drop, len(cand))
+    dropped = rng.choice(cand, n_drop, replace=False)
+    keep = np.ones(len(e), bool)
+    keep[dropped] = False
+    e = e[keep]
+    w = np.rint(scale * 1.0 * rng.uniform(1.0, 1.5, len(e))).astype(np.int32)
+    yy, xx = np.divmod(np.arange(rows * cols), cols)
+    xy = np.stack([xx, yy], 1).astype(np.float64)
+    return Graph(rows * cols, e[:, 0], e[:, 1], w, xy)
+
+
+def delaunay_graph(n, avg_degree=2.8, seed=1, scale=1e4):
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    xy = rng.random((n, 2))
+    tri = Delaunay(xy)
+    s = tri.simplices
+    e = np.concatenate([s[:, [0, 1]], s[:, [1, 2]], s[:, [0, 2]]])
+    e = np.sort(e, 1)
+    e = np.unique(e, axis=0)
+    length = np.linalg.norm(xy[e[:, 0]] - xy[e[:, 1]], axis=1)
+    tree = _spanning_tree_scipy(n, e[:, 0], e[:, 1], length)
+    target = int(round(avg_degree * n / 2))
+    extra_needed = max(0, target - (n - 1))
+    cand = np.flatnonzero(~tree)
+    extra = rng.choice(cand, min(extra_needed, len(cand)), replace=False)
+    keep = tree.copy()
+    keep[extra] = True
+    e = e[keep]
+    length = length[keep]
+    w = np.maximum(1, np.rint(scale * length * rng.uniform(1.0, 1.5, len(e)))).astype(np.int32)
+    return Graph(n, e[:, 0], e[:, 1], w, xy)
+
+
+def bisect(xy, height):
+    """Recursive coordinate bisection.  Returns (index_ids, cut) where index_ids
+    is the ID array (index -> vertex id) and cut is a (2^H - 1, 2) int32 heap of
+    [l, r) ranges, heap node i at level floor(log2(i+1)) + 1."""
+    n = len(xy)
+    order = np.arange(n)
+    full = (1 << height) - 1
+    cut = np.zeros((full, 2), np.int32)
+    cut[0] = (0, n)
+    for i in range(full):
+        level = int(np.floor(np.log2(i + 1)))
+        l, r = int(cut[i, 0]), int(cut[i, 1])
+        seg = order[l:r]
+        axis = level % 2
+        key = xy[seg, axis]
+        perm = np.lexsort((seg, key))
+        order[l:r] = seg[perm]
+        c1 = 2 * i + 1
+        if c1 < full:
+            mid = (l + r) // 2
+            cut[c1] = (l, mid)
+            cut[c1 + 1] = (mid, r)
+    return order.astype(np.int32), cut
+
+
+def write_files(g, index_ids, cut, height, root, name, method="Latitude"):
+    """Writes ./Graph/USA/<name>{Edge,Node}.txt and ./Partition/<name>/<method>{ID,Cut}.txt
+    under `root`, in the reference's formats."""
+    gdir = os.path.join(root, "Graph", "USA")
+    pdir = os.path.join(root, "Partition", name)
+    os.makedirs(gdir, exist_ok=True)
+    os.makedirs(pdir, exist_ok=True)
+    u, v, w = g.arcs()
+    with open(os.path.join(gdir, f"{name}Edge.txt"), "w") as f:
+        f.write(f"{g.n} {len(u)}\n")
+        f.write("".join(f"a {a + 1} {b + 1} {c}\n" for a, b, c in zip(u.tolist(), v.tolist(), w.tolist())))
+    with open(os.path.join(gdir, f"{name}Node.txt"), "w") as f:
+        f.write(f"{g.n}\n")
+        f.write("".join(f"v {i + 1} {x!r} {y!r}\n" for i, (x, y) in enumerate(g.xy.tolist())))
+    with open(os.path.join(pdir, f"{method}ID.txt"), "w") as f:
+        f.write(f"{g.n}\n")
+        f.write("".join(f"{i} {g.xy[vid, 0]!r} {g.xy[vid, 1]!r} {vid}\n" for i, vid in enumerate(index_ids.tolist())))
+    with open(os.path.join(pdir, f"{method}Cut.txt"), "w") as f:
+        f.write(f"{height}\n")
+        lines = []
+        for i in range(len(cut)):
+            level = int(np.floor(np.log2(i + 1))) + 1
+            lines.append(f"{level} {cut[i, 0]} {cut[i, 1]}\n")
+        f.write("".join(lines))
+
+
+# Workloads: small grid (4096 V, H=4) and the road-like Delaunay sizes.
+WORKLOADS = {
+    "grid64": dict(kind="grid", rows=64, cols=64, drop=0.10, seed=1, height=4),
+    "ny264k": dict(kind="delaunay", n=264346, avg_degree=2.8, seed=2, height=10),
+    "fla1m": dict(kind="delaunay", n=1070376, avg_degree=2.5, seed=3, height=12),
+    "e3m6": dict(kind="delaunay", n=3598623, avg_degree=2.4, seed=4, height=14),
+}
+
+
+def make_workload(name):
+    spec = dict(WORKLOADS[name])
+    kind = spec.pop("kind")
+    height = spec.pop("height")
+    if kind == "grid":
+        g = grid_graph(**spec)
+    else:
+        g = delaunay_graph(**spec)
+    ids, cut = bisect(g.xy, height)
+    return g, ids, cut, height

@@ -1,0 +1,1195 @@
+// h2h_oracle.cpp -- CPU restatement of the reference GPU-H2H hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+// product library (gpu-h2h_amd/) never links, loads or calls anything here.
+//
+// Every function restates one reference function; the citation names the
+// reference file:line it follows (sauccjy/GPU-H2H).  Data structures follow
+// the reference where they change results (std::set heap keyed by HostRank,
+// unordered_map adjacency, stable_sort of candidates, the ragged RMQ layout),
+// so the outputs are those of the reference binary on the same inputs.
+//
+// Pinning (what checks each function independently of this text) is listed
+// in tests/test_oracle_pins.py; the literal restatement of the GPU partition
+// kernel (orc_contract with mode ORC_GPU_LITERAL) pins the fast one.
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <queue>
+#include <set>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+namespace {
+
+// pairs (Graph_D.cuh:86-133)
+struct Pairs {
+  int first = -1;
+  int second = -1;
+  Pairs() = default;
+  Pairs(int f, int s) : first(f), second(s) {}
+};
+
+// TDrank (Graph_D.cuh:135-207): first = degree, second = id, third = height.
+// operator< orders by (third+first, first, second) (Graph_D.cuh:154-173).
+struct TDrank {
+  int first = -1;
+  int second = -1;
+  int third = 0;
+  TDrank() = default;
+  TDrank(int f, int s, int t) : first(f), second(s), third(t) {}
+  bool operator<(const TDrank& o) const {
+    if (third + first != o.third + o.first) return third + first < o.third + o.first;
+    if (first != o.first) return first < o.first;
+    return second < o.second;
+  }
+};
+
+// myPair<int> (Graph_D.cuh:232-264): NodeID = hub, first = neighbour, second = weight.
+// operator< compares first only (Graph_D.cuh:253-255).
+struct MyPair {
+  int NodeID = -1;
+  int first = 0;
+  int second = 0;
+  MyPair() = default;
+  MyPair(int f, int s, int id) : NodeID(id), first(f), second(s) {}
+  void setPairs(int id, int f, int s) { NodeID = id; first = f; second = s; }
+  bool operator<(const MyPair& o) const { return first < o.first; }
+};
+
+// HostRank (Graph_D.cuh:212-230): order by (deg+height, deg, id) over CHInfo.
+struct HostRankLess {
+  const std::vector<Pairs>* info;
+  bool operator()(int a, int b) const {
+    const Pairs& x = (*info)[a];
+    const Pairs& y = (*info)[b];
+    if (x.first + x.second != y.first + y.second) return x.first + x.second < y.first + y.second;
+    if (x.first != y.first) return x.first < y.first;
+    return a < b;
+  }
+};
+
+using Clock = std::chrono::steady_clock;
+static long long usec_since(Clock::time_point t0) {
+  return std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() - t0).count();
+}
+
+struct Oracle {
+  // ---- inputs (Graph_D.cuh:285-330) ----
+  int NodeNumber = 0;
+  long long EdgeNumber = 0;
+  int threadNumber = 1;
+  int TreeHeight = 1;
+  int changeHeight = 1;
+  std::vector<std::vector<Pairs>> adjList;  // (neighbour, weight), as listed
+  std::vector<int> NE_P_id;                 // NE_P[index].NodeID
+  std::vector<int> ID_hash;                 // vertex -> index
+  std::vector<Pairs> partition_Tree;        // heap of [l, r)
+
+  // ---- partition rank tree (GraphPartition.cu) ----
+  std::vector<TDrank> nonAdjcentNode;
+  std::vector<MyPair> NANHash;  // first = start, second = max size, NodeID = count
+  std::vector<double> layerFraction;
+  std::vector<MyPair> CHTreeHash;  // first = size, second = capacity, NodeID = offset
+  std::vector<MyPair> CHTree;
+  long long lubLength = 0;
+
+  // ---- contraction ----
+  std::vector<Pairs> CHInfo_H;  // device CHInfo mirror (deg, height)
+  std::vector<Pairs> CHInfo;    // Traditional_H2H.cu static CHInfo
+  std::vector<int> CHChild;
+  std::vector<std::unordered_map<int, Pairs>> CHAdjlist;  // nbr -> (weight, hub)
+  std::vector<char> visited;
+  int tempnonAdjcentNodeIndex = INT_MAX;
+  long long vertexesConstructedInGPU = 0;
+
+  // ---- tree (TreeStructure.cu) ----
+  std::vector<int> ranks;
+  int head = -1;
+  std::vector<int> father;
+  std::vector<Pairs> ChildHash;  // (count, offset)
+  std::vector<int> Childs;
+  std::vector<Pairs> OULA_DFS_;  // (node, depth)
+  std::vector<int> firstAppeare;
+  std::vector<int> H2H_TreeHeight_Hash;
+  long long H2H_Size = 0;
+  int CHTreeHeight = 0;
+  std::vector<int64_t> RMQHash;
+  std::vector<int> RMQ_ID;
+
+  // ---- labels (H2HConsturction.cu) ----
+  std::vector<int64_t> H2H_pos_hash, H2H_dis_hash;
+  std::vector<int> H2H_pos_ID, H2H_pos_POS, H2H_dis;
+  std::vector<int> TreeHash, TreeBFS;
+  std::vector<int64_t> TreeBFS_Hash;
+  std::vector<int> TreeBFS_ID, TreeBFS_adj, TreeBFS_pos;
+
+  // ---- timings (usec) ----
+  long long t_partition = 0, t_contract_partition = 0, t_contract_top = 0, t_tree = 0,
+            t_malloc = 0, t_bfs = 0, t_label = 0;
+
+  std::string err;
+};
+
+static int fail(Oracle* g, const std::string& msg) {
+  if (g) g->err = msg;
+  return -1;
+}
+
+// ---------------------------------------------------------------------------
+// Loader.  Graph_Fundamental.cu:4-95.  Arcs are given 0-based; self-loops are
+// dropped and arcs are stored directed as listed (Graph_Fundamental.cu:46-55).
+// The cut heap holds the first 2^min(Hfile,Harg)-1 lines (:64-78).
+// CHInfo_H[i] = (adjList[i].size(), 0) (:85-91).
+// ---------------------------------------------------------------------------
+static void finish_load(Oracle* g) {
+  g->CHInfo_H.assign(g->NodeNumber, Pairs(-1, 0));
+  for (int i = 0; i < g->NodeNumber; i++) g->CHInfo_H[i].first = (int)g->adjList[i].size();
+  g->visited.assign(g->NodeNumber, 0);
+  g->CHAdjlist.assign(g->NodeNumber, std::unordered_map<int, Pairs>());
+}
+
+// ---------------------------------------------------------------------------
+// makeAdjcentNode.  GraphPartition.cu:6-136 (with goalHeight = 0, main.cu:91).
+// ---------------------------------------------------------------------------
+static void make_adjcent_node(Oracle* g) {
+  auto t0 = Clock::now();
+  const int N = g->NodeNumber;
+  int tempHeight = g->TreeHeight;
+  std::vector<char> isAdjcent(N, 1);
+  std::vector<int> placeOccupy(N, 0);
+  for (int i = 0; i < N; i++) placeOccupy[i] = (int)g->adjList[i].size();
+  g->NANHash.assign(g->partition_Tree.size(), MyPair());
+  g->nonAdjcentNode.clear();
+  std::vector<double> afs(g->TreeHeight + 1, 0);
+  while (tempHeight > 0) {
+    long long lo = (1LL << (tempHeight - 1)) - 1;
+    long long hi = (1LL << tempHeight) - 2;
+    std::vector<char> isAdjcent_cp = isAdjcent;
+    for (long long i = hi; i >= lo; i--) {
+      g->NANHash[i].first = (int)g->nonAdjcentNode.size();
+      g->NANHash[i].second = 0;
+      g->NANHash[i].NodeID = 0;
+      int left = g->partition_Tree[i].first, right = g->partition_Tree[i].second;
+      std::map<int, int> subgraph;
+      for (int j = left; j < right; j++) {
+        if (!isAdjcent[g->NE_P_id[j]]) continue;
+        subgraph.emplace(g->NE_P_id[j], 0);
+      }
+      g->NANHash[i].second = (int)subgraph.size();
+      for (auto& it : subgraph) {
+        it.second = (int)subgraph.size();
+        for (auto& adj : g->adjList[it.first]) {
+          if (!isAdjcent[adj.first]) continue;
+          if (subgraph.find(adj.first) == subgraph.end()) it.second++;
+        }
+      }
+      for (auto& it : subgraph) {
+        if (it.second == g->NANHash[i].second) {
+          isAdjcent_cp[it.first] = 0;
+          g->NANHash[i].NodeID++;
+          g->nonAdjcentNode.push_back(TDrank((int)g->adjList[it.first].size(), it.first, 0));
+        }
+        placeOccupy[it.first] = std::max(placeOccupy[it.first], it.second);
+      }
+    }
+    isAdjcent = isAdjcent_cp;
+    afs[tempHeight] = (double)g->nonAdjcentNode.size();
+    tempHeight--;
+  }
+  g->layerFraction.assign(g->TreeHeight, 0.0);
+  for (int i = 1; i <= g->TreeHeight; i++) g->layerFraction[i - 1] = afs[i] / N;
+  g->t_partition = usec_since(t0);
+
+  // LUB allocation (GraphPartition.cu:267-302)
+  g->CHTree.clear();
+  g->CHTreeHash.assign(N, MyPair());
+  long long total = 0;
+  for (int i = 0; i < N; i++) total += placeOccupy[g->NE_P_id[i]];
+  g->CHTree.reserve(total);
+  for (int i = 0; i < N; i++) {
+    int nodeID = g->NE_P_id[i];
+    g->CHTreeHash[i].first = 0;
+    g->CHTreeHash[i].second = placeOccupy[nodeID];
+    g->CHTreeHash[i].NodeID = (int)g->CHTree.size();
+    g->CHTree.insert(g->CHTree.end(), g->CHTreeHash[i].second, MyPair(INT_MAX, 0, -1));
+    for (auto& it : g->adjList[nodeID]) {
+      g->CHTree[g->CHTreeHash[i].NodeID + g->CHTreeHash[i].first].setPairs(-1, it.first, it.second);
+      g->CHTreeHash[i].first++;
+    }
+  }
+  g->lubLength = (long long)g->CHTree.size();
+}
+
+// ---------------------------------------------------------------------------
+// Literal restatement of kernel makeCHPerHeight_D_2 (kernel_functions.cu:310-479)
+// and its level loop makeCHPerHeight_2 (:520-553), one partition at a time
+// (partitions of one level touch disjoint slots, so the serial order over
+// them is immaterial).  bobbleSortPairs :21-41, bobbleSortmyPair :65-85.
+// O(n^3) per partition: small inputs only.
+// ---------------------------------------------------------------------------
+static void bubble_tdrank(std::vector<TDrank>& a, int left, int right) {
+  for (int i = left; i <= right; i++)
+    for (int j = left; j <= right - 1; j++)
+      if (a[j + 1] < a[j]) std::swap(a[j], a[j + 1]);
+}
+static void bubble_mypair(std::vector<MyPair>& a, int left, int right) {
+  for (int i = left; i <= right; i++) {
+    bool swapped = false;
+    for (int j = left; j <= right - 1; j++)
+      if (a[j + 1] < a[j]) { std::swap(a[j], a[j + 1]); swapped = true; }
+    if (!swapped) break;
+  }
+}
+
+static int gpu_partition_literal(Oracle* g, long long part, std::vector<Pairs>& CHInfo) {
+  auto& nan = g->nonAdjcentNode;
+  auto& T = g->CHTree;
+  auto& TH = g->CHTreeHash;
+  const int s = g->NANHash[part].first;
+  const int cnt = g->NANHash[part].NodeID;
+  for (int j = s; j < s + cnt; j++) {
+    bubble_tdrank(nan, j, s + cnt - 1);
+    int v = nan[j].second;
+    g->visited[v] = 1;
+    int hi = g->ID_hash[v];
+    int off = TH[hi].NodeID;
+    int sz = TH[hi].first;
+    bubble_mypair(T, off, off + sz - 1);
+    for (int k = off + 1; k < off + sz; k++) {
+      if (T[k].first == T[k - 1].first && T[k].first != INT_MAX) {
+        if (T[k - 1].second < T[k].second) {
+          T[k].second = T[k - 1].second;
+          T[k].NodeID = T[k - 1].NodeID;
+        }
+        T[k - 1].first = INT_MAX;
+      }
+    }
+    bubble_mypair(T, off, off + sz - 1);
+    for (int k = off; k < off + sz; k++) {
+      if (T[k].first == INT_MAX) break;
+      TH[hi].first = k - off + 1;
+    }
+    sz = TH[hi].first;
+    // delete v from its neighbours' slot lists
+    for (int k = off; k < off + sz; k++) {
+      int o = T[k].first;
+      int ohi = g->ID_hash[o];
+      int ooff = TH[ohi].NodeID;
+      int osz = TH[ohi].first;
+      for (int q = TH[ohi].NodeID; q < TH[ohi].NodeID + TH[ohi].first; q++)
+        if (T[q].first == v) T[q].first = INT_MAX;
+      bubble_mypair(T, ooff, ooff + osz - 1);
+      for (int k2 = TH[ohi].NodeID; k2 < ooff + osz; k2++) {
+        if (T[k2].first == INT_MAX) break;
+        TH[ohi].first = k2 - TH[ohi].NodeID + 1;
+      }
+    }
+    // fill-in between every neighbour pair
+    for (int k = off; k < off + sz; k++) {
+      int L = T[k].first;
+      for (int q = k + 1; q < off + sz; q++) {
+        int R = T[q].first;
+        if (L == R) continue;
+        int length = T[k].second + T[q].second;
+        bool leftHaveRight = false;
+        int lh = g->ID_hash[L], rh = g->ID_hash[R];
+        for (int p = TH[lh].NodeID; p < TH[lh].NodeID + TH[lh].first; p++) {
+          if (T[p].first == R) {
+            leftHaveRight = true;
+            if (T[p].second > length) { T[p].second = length; T[p].NodeID = v; }
+            break;
+          }
+        }
+        for (int p = TH[rh].NodeID; p < TH[rh].NodeID + TH[rh].first; p++) {
+          if (T[p].first == L) {
+            if (T[p].second > length) { T[p].second = length; T[p].NodeID = v; }
+            break;
+          }
+        }
+        if (leftHaveRight) continue;
+        if (TH[lh].first >= TH[lh].second || TH[rh].first >= TH[rh].second)
+          return fail(g, "LUB slot capacity exceeded in literal GPU contraction");
+        int e = TH[lh].NodeID + TH[lh].first;
+        T[e].NodeID = v; T[e].first = R; T[e].second = length;
+        TH[lh].first++;
+        e = TH[rh].NodeID + TH[rh].first;
+        T[e].NodeID = v; T[e].first = L; T[e].second = length;
+        TH[rh].first++;
+      }
+    }
+    // degree / height of the neighbours, then refresh the remaining keys
+    for (int q = off; q < off + sz; q++) {
+      if (T[q].first == INT_MAX) continue;
+      int o = T[q].first;
+      int HeightNow = nan[j].third;
+      CHInfo[o].first = TH[g->ID_hash[o]].first;
+      CHInfo[o].second = std::max(HeightNow + 1, CHInfo[o].second);
+    }
+    for (int k = j + 1; k < s + cnt; k++) {
+      nan[k].first = TH[g->ID_hash[nan[k].second]].first;
+      nan[k].third = CHInfo[nan[k].second].second;
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fast restatement of the same kernel on unordered_map adjacency.  Identical
+// results on simple symmetric graphs: the kernel's slot list of a vertex is
+// its current neighbour set with the minimum weight per neighbour; the only
+// departure from the CPU rule (Traditional_H2H.cu:411-465) is that the first
+// pick of each partition sorts on the keys makeAdjcentNode wrote (original
+// degree, height 0; GraphPartition.cu:236) and propagates HeightNow = 0
+// (kernel_functions.cu:325-329, 466-476): the keys are refreshed from the
+// slots only after each elimination (:472-476).
+// ---------------------------------------------------------------------------
+// holes != nullptr selects the slot-list degree of the GPU kernel: deleting
+// the last real entry of a neighbour's slot list leaves its size unchanged
+// (the size is only recomputed when a real entry survives the sort,
+// kernel_functions.cu:376-388), so that slot stays counted in the degree
+// key (:473) until the next deletion compacts the list.
+static void eliminate_cpu_rule(Oracle* g, int v, int heightNow, std::vector<int>* holes = nullptr) {
+  // Traditional_H2H.cu:414-465 (the body shared by partition_TD_T and CHUsingHeap_2)
+  auto& adj = g->CHAdjlist;
+  std::vector<Pairs> info;
+  std::vector<int> nei;
+  for (auto it = adj[v].begin(); it != adj[v].end(); ++it) {
+    nei.push_back(it->first);
+    info.push_back(it->second);
+  }
+  for (size_t a = 0; a < nei.size(); a++) {
+    auto it = adj[nei[a]].find(v);
+    if (it != adj[nei[a]].end()) adj[nei[a]].erase(it);
+    if (holes) (*holes)[nei[a]] = adj[nei[a]].empty() ? (*holes)[nei[a]] + 1 : 0;
+  }
+  for (size_t a = 0; a < nei.size(); a++) {
+    int L = nei[a];
+    for (size_t b = a + 1; b < nei.size(); b++) {
+      int R = nei[b];
+      int weight = info[a].first + info[b].first;
+      auto fl = adj[L].find(R);
+      if (fl == adj[L].end()) adj[L].emplace(R, Pairs(weight, v));
+      else if (fl->second.first > weight) { fl->second.first = weight; fl->second.second = v; }
+      auto fr = adj[R].find(L);
+      if (fr == adj[R].end()) adj[R].emplace(L, Pairs(weight, v));
+      else if (fr->second.first > weight) { fr->second.first = weight; fr->second.second = v; }
+    }
+  }
+  for (size_t a = 0; a < nei.size(); a++) {
+    int u = nei[a];
+    int height = std::max(heightNow + 1, g->CHInfo[u].second);
+    if (height == g->CHInfo[u].second) g->CHChild[u] += g->CHChild[v];
+    else g->CHChild[u] = 1 + g->CHChild[v];
+    g->CHInfo[u] = Pairs((int)adj[u].size() + (holes ? (*holes)[u] : 0), height);
+  }
+}
+
+static void load_adjlist_into_map(Oracle* g) {
+  // Traditional_H2H.cu:369-376
+  for (int i = 0; i < g->NodeNumber; i++) {
+    for (auto& it : g->adjList[i]) g->CHAdjlist[i].emplace(it.first, Pairs(it.second, -1));
+    g->CHInfo[i] = Pairs((int)g->adjList[i].size(), 0);
+  }
+}
+
+static void gpu_partition_fast(Oracle* g, long long part, std::vector<int>& holes) {
+  const int s = g->NANHash[part].first;
+  const int cnt = g->NANHash[part].NodeID;
+  std::vector<TDrank> cand(g->nonAdjcentNode.begin() + s, g->nonAdjcentNode.begin() + s + cnt);
+  for (int j = 0; j < cnt; j++) {
+    if (j > 0)
+      for (int k = j; k < cnt; k++) {
+        int id = cand[k].second;
+        cand[k].first = (int)g->CHAdjlist[id].size() + holes[id];
+        cand[k].third = g->CHInfo[id].second;
+      }
+    int best = j;
+    for (int k = j + 1; k < cnt; k++)
+      if (cand[k] < cand[best]) best = k;
+    std::swap(cand[j], cand[best]);
+    int v = cand[j].second;
+    g->visited[v] = 1;
+    g->nonAdjcentNode[s + j] = cand[j];
+    eliminate_cpu_rule(g, v, cand[j].third, &holes);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// uploadCHset.  Traditional_H2H.cu:28-118.
+// ---------------------------------------------------------------------------
+static void upload_chset(Oracle* g, int tempheight) {
+  g->CHInfo.assign(g->NodeNumber, Pairs());
+  g->CHChild.assign(g->NodeNumber, 0);
+  for (auto& m : g->CHAdjlist) m.clear();
+  std::vector<int> topSet;  // insertion is into the std::set in CHUsingHeap_2
+  int tempHeight = tempheight;
+  while (tempHeight > 0) {
+    long long lo = (1LL << (tempHeight - 1)) - 1;
+    long long hi = (1LL << tempHeight) - 2;
+    tempHeight--;
+    for (long long i = hi; i >= lo; i--) {
+      int startIndex = g->NANHash[i].first;
+      int TDNodeSize = g->NANHash[i].NodeID;
+      g->tempnonAdjcentNodeIndex = std::min(startIndex, g->tempnonAdjcentNodeIndex);
+      for (int j = startIndex; j < startIndex + TDNodeSize; j++) {
+        int v = g->nonAdjcentNode[j].second;
+        int hx = g->ID_hash[v];
+        int off = g->CHTreeHash[hx].NodeID, sz = g->CHTreeHash[hx].first;
+        int size = 0;
+        for (int k = off; k < off + sz; k++) {
+          if (g->CHTree[k].first == INT_MAX) continue;
+          size++;
+          g->CHAdjlist[v].emplace(g->CHTree[k].first, Pairs(g->CHTree[k].second, g->CHTree[k].NodeID));
+        }
+        g->CHInfo[v] = Pairs(size, g->CHInfo_H[v].second);
+        topSet.push_back(v);
+      }
+    }
+  }
+  tempHeight = g->TreeHeight;
+  while (tempHeight > tempheight) {
+    long long lo = (1LL << (tempHeight - 1)) - 1;
+    long long hi = (1LL << tempHeight) - 2;
+    tempHeight--;
+    for (long long i = hi; i >= lo; i--) {
+      int startIndex = g->NANHash[i].first;
+      int TDNodeSize = g->NANHash[i].NodeID;
+      for (int j = startIndex; j < startIndex + TDNodeSize; j++) {
+        int v = g->nonAdjcentNode[j].second;
+        int hx = g->ID_hash[v];
+        int off = g->CHTreeHash[hx].NodeID, sz = g->CHTreeHash[hx].first;
+        for (int k = off; k < off + sz; k++) {
+          if (g->CHTree[k].first == INT_MAX) continue;
+          g->CHAdjlist[v].emplace(g->CHTree[k].first, Pairs(g->CHTree[k].second, g->CHTree[k].NodeID));
+        }
+      }
+    }
+  }
+  g->CHTree.clear(); g->CHTree.shrink_to_fit();
+  // the heap contents are rebuilt in chusing_heap_2 from the top levels
+}
+
+static std::vector<int> top_vertices(Oracle* g, int tempheight) {
+  std::vector<int> out;
+  for (int h = tempheight; h > 0; h--) {
+    long long lo = (1LL << (h - 1)) - 1, hi = (1LL << h) - 2;
+    for (long long i = hi; i >= lo; i--)
+      for (int j = g->NANHash[i].first; j < g->NANHash[i].first + g->NANHash[i].NodeID; j++)
+        out.push_back(g->nonAdjcentNode[j].second);
+  }
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// CHUsingHeap_2.  Traditional_H2H.cu:216-351.
+// ---------------------------------------------------------------------------
+static void chusing_heap_2(Oracle* g, const std::vector<int>& tops) {
+  HostRankLess less{&g->CHInfo};
+  std::set<int, HostRankLess> heap(less);
+  for (int v : tops) heap.insert(v);
+  long long tempNANIndex = g->tempnonAdjcentNodeIndex;
+  g->vertexesConstructedInGPU = g->tempnonAdjcentNodeIndex;
+  auto& adj = g->CHAdjlist;
+  while (!heap.empty()) {
+    int v = *heap.begin();
+    heap.erase(heap.begin());
+    g->visited[v] = 1;
+    std::vector<Pairs> info;
+    std::vector<int> nei;
+    g->nonAdjcentNode[tempNANIndex++] = TDrank((int)adj[v].size(), v, g->CHInfo[v].second);
+    for (auto it = adj[v].begin(); it != adj[v].end(); ++it) {
+      nei.push_back(it->first);
+      info.push_back(it->second);
+    }
+    for (size_t a = 0; a < nei.size(); a++) {
+      auto it = adj[nei[a]].find(v);
+      if (it != adj[nei[a]].end()) adj[nei[a]].erase(it);
+      heap.erase(nei[a]);
+    }
+    for (size_t a = 0; a < nei.size(); a++) {
+      int L = nei[a];
+      for (size_t b = a + 1; b < nei.size(); b++) {
+        int R = nei[b];
+        int weight = info[a].first + info[b].first;
+        auto fl = adj[L].find(R);
+        if (fl == adj[L].end()) adj[L].emplace(R, Pairs(weight, v));
+        else if (fl->second.first > weight) { fl->second.first = weight; fl->second.second = v; }
+        auto fr = adj[R].find(L);
+        if (fr == adj[R].end()) adj[R].emplace(L, Pairs(weight, v));
+        else if (fr->second.first > weight) { fr->second.first = weight; fr->second.second = v; }
+      }
+    }
+    for (size_t a = 0; a < nei.size(); a++) {
+      int u = nei[a];
+      int height = std::max(g->CHInfo[v].second + 1, g->CHInfo[u].second);
+      if (height == g->CHInfo[u].second) g->CHChild[u] += g->CHChild[v] + 1;
+      else g->CHChild[u] = 1 + g->CHChild[v];
+      g->CHInfo[u] = Pairs((int)adj[u].size(), height);
+      heap.insert(u);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// partition_TD_T.  Traditional_H2H.cu:354-496 (threads own chunks of
+// consecutive partitions; each picks the minimum over its whole chunk).
+// ---------------------------------------------------------------------------
+static void partition_td_t(Oracle* g, int height) {
+  g->CHTree.clear(); g->CHTree.shrink_to_fit();
+  g->CHInfo.assign(g->NodeNumber, Pairs());
+  g->CHChild.assign(g->NodeNumber, 0);
+  load_adjlist_into_map(g);
+  int tempHeight = g->TreeHeight;
+  const int T = g->threadNumber;
+  HostRankLess less{&g->CHInfo};
+  while (tempHeight > height) {
+    long long lo = (1LL << (tempHeight - 1)) - 1;
+    long long hiEx = (1LL << tempHeight) - 1;
+    long long totalSubGraph = hiEx - lo;
+    long long maxPer = totalSubGraph / T + 1;
+    std::vector<std::vector<int>> cand(T);
+    std::vector<int> range(T, INT_MAX);
+    long long it = 0, temp = 0;
+    for (long long i = lo; i < hiEx; i++) {
+      int left = g->NANHash[i].first, cnt = g->NANHash[i].NodeID;
+      for (int j = left; j < left + cnt; j++) cand[it].push_back(g->nonAdjcentNode[j].second);
+      range[it] = std::min(range[it], left);
+      if (++temp == maxPer) { it++; temp = 0; }
+    }
+    std::vector<std::thread> threads;
+    for (int t = 0; t < T; t++) {
+      threads.emplace_back([g, &cand, &range, t, less]() {
+        auto& c = cand[t];
+        while (!c.empty()) {
+          std::stable_sort(c.begin(), c.end(), less);
+          int v = c.front();
+          g->nonAdjcentNode[range[t]] = TDrank((int)g->CHAdjlist[v].size(), v, g->CHInfo[v].second);
+          range[t]++;
+          eliminate_cpu_rule(g, v, g->CHInfo[v].second);
+          c.erase(c.begin());
+        }
+      });
+    }
+    for (auto& th : threads) th.join();
+    tempHeight--;
+  }
+  while (tempHeight > 0) {
+    long long lo = (1LL << (tempHeight - 1)) - 1;
+    long long hiEx = (1LL << tempHeight) - 1;
+    tempHeight--;
+    for (long long i = lo; i < hiEx; i++)
+      g->tempnonAdjcentNodeIndex = std::min(g->NANHash[i].first, g->tempnonAdjcentNodeIndex);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tree: getrank (TreeStructure.cu:102-119), getTreeStructure (:121-163),
+// OULADFS (:242-262, iterative here with the same visit order) and
+// makeOULAAndRMQ (:190-240).
+// ---------------------------------------------------------------------------
+static void build_tree(Oracle* g) {
+  auto t0 = Clock::now();
+  const int N = g->NodeNumber;
+  g->ranks.assign(N, -1);
+  for (size_t i = 0; i < g->nonAdjcentNode.size(); i++) g->ranks[g->nonAdjcentNode[i].second] = (int)i;
+  g->head = g->nonAdjcentNode.back().second;
+  g->father.assign(N, -1);
+  g->ChildHash.assign(N, Pairs(0, -1));
+  std::vector<int> pos(N, 0);
+  g->Childs.assign(N - 1 > 0 ? N - 1 : 0, -1);
+  for (int i = 0; i < N; i++) {
+    int f = -1, r = INT_MAX;
+    for (auto& it : g->CHAdjlist[i])
+      if (r > g->ranks[it.first]) { r = g->ranks[it.first]; f = it.first; }
+    g->father[i] = f;
+  }
+  for (int i = 0; i < N; i++)
+    if (g->father[i] != -1) g->ChildHash[g->father[i]].first++;
+  int temp = 0;
+  for (int i = 0; i < N; i++) { g->ChildHash[i].second = temp; temp += g->ChildHash[i].first; }
+  for (int i = 0; i < N; i++) {
+    if (g->father[i] == -1) continue;
+    g->Childs[g->ChildHash[g->father[i]].second + pos[g->father[i]]] = i;
+    pos[g->father[i]]++;
+  }
+  // Euler tour
+  g->OULA_DFS_.clear();
+  g->firstAppeare.assign(N, -1);
+  g->H2H_TreeHeight_Hash.assign(N, -1);
+  g->H2H_Size = 0;
+  g->CHTreeHeight = 0;
+  std::vector<char> visit(N, 0);
+  struct Frame { int node, depth, next; };
+  std::vector<Frame> st;
+  auto enter = [&](int v, int d) {
+    g->OULA_DFS_.push_back(Pairs(v, d));
+    g->firstAppeare[v] = (int)g->OULA_DFS_.size() - 1;
+    g->H2H_TreeHeight_Hash[v] = d;
+    g->H2H_Size += d;
+    visit[v] = 1;
+    g->CHTreeHeight = std::max(g->CHTreeHeight, d);
+    st.push_back(Frame{v, d, g->ChildHash[v].second});
+  };
+  enter(g->head, 1);
+  while (!st.empty()) {
+    Frame& f = st.back();
+    int end = g->ChildHash[f.node].second + g->ChildHash[f.node].first;
+    if (f.next < end) {
+      int c = g->Childs[f.next++];
+      if (!visit[c]) enter(c, f.depth + 1);
+    } else {
+      st.pop_back();
+      if (!st.empty()) g->OULA_DFS_.push_back(Pairs(st.back().node, st.back().depth));
+    }
+  }
+  // sparse table (depth, id); compacted into ragged RMQHash / RMQ_ID
+  const long long S = (long long)g->OULA_DFS_.size();
+  int jSize = (int)(std::log2((double)S));
+  int L = jSize + 1;
+  std::vector<Pairs> one((size_t)(L * S), Pairs());
+  for (long long i = 0; i < S; i++) {
+    one[i * L].first = g->OULA_DFS_[i].second;
+    one[i * L].second = g->OULA_DFS_[i].first;
+  }
+  for (int j = 1; j <= jSize; j++) {
+    long long dv = 1LL << (j - 1);
+    for (long long i = 0; i + dv < S; i++) {
+      if (one[i * L + j - 1].first <= one[(i + dv) * L + j - 1].first) one[i * L + j] = one[i * L + j - 1];
+      else one[i * L + j] = one[(i + dv) * L + j - 1];
+    }
+  }
+  g->RMQHash.assign(S, -1);
+  g->RMQ_ID.clear();
+  for (long long i = 0; i < S; i++) {
+    g->RMQHash[i] = (int64_t)g->RMQ_ID.size();
+    for (int j = 0; j < L; j++) {
+      if (one[i * L + j].first == -1) break;
+      g->RMQ_ID.push_back(one[i * L + j].second);
+    }
+  }
+  g->t_tree = usec_since(t0);
+}
+
+// ---------------------------------------------------------------------------
+// mallocH2HLabel_noHub (H2HConsturction.cu:618-722) and constructBFSTree
+// (:324-377).
+// ---------------------------------------------------------------------------
+static void malloc_labels(Oracle* g) {
+  auto t0 = Clock::now();
+  const int N = g->NodeNumber;
+  g->H2H_pos_hash.assign(1, 0);
+  g->H2H_dis_hash.assign(1, 0);
+  g->H2H_dis.clear(); g->H2H_pos_ID.clear(); g->H2H_pos_POS.clear();
+  g->H2H_dis.reserve(g->H2H_Size);
+  for (int i = 0; i < N; i++) {
+    int d = g->OULA_DFS_[g->firstAppeare[i]].second;
+    g->H2H_dis.insert(g->H2H_dis.end(), d, INT_MAX);
+    size_t start = g->H2H_pos_ID.size();
+    for (auto& adj : g->CHAdjlist[i]) {
+      int a = adj.first;
+      int p = g->OULA_DFS_[g->firstAppeare[a]].second - 1;
+      g->H2H_pos_ID.push_back(a);
+      g->H2H_pos_POS.push_back(p);
+      g->H2H_dis[g->H2H_dis_hash[i] + (int64_t)p] = adj.second.first;
+    }
+    g->H2H_pos_ID.push_back(i);
+    g->H2H_pos_POS.push_back(d - 1);
+    g->H2H_dis[g->H2H_dis.size() - 1] = 0;
+    g->H2H_pos_hash.push_back((int64_t)g->H2H_pos_ID.size());
+    g->H2H_dis_hash.push_back((int64_t)g->H2H_dis.size());
+    // sort_by_key(POS, ID) over this vertex's entries (keys unique)
+    std::vector<std::pair<int, int>> kv;
+    for (size_t k = start; k < g->H2H_pos_ID.size(); k++) kv.emplace_back(g->H2H_pos_POS[k], g->H2H_pos_ID[k]);
+    std::sort(kv.begin(), kv.end());
+    for (size_t k = 0; k < kv.size(); k++) {
+      g->H2H_pos_POS[start + k] = kv[k].first;
+      g->H2H_pos_ID[start + k] = kv[k].second;
+    }
+  }
+  g->t_malloc = usec_since(t0);
+  t0 = Clock::now();
+  // constructBFSTree
+  g->TreeHash.assign(1, 0);
+  g->TreeBFS.assign(1, g->head);
+  g->TreeHash.push_back((int)g->TreeBFS.size());
+  g->TreeBFS_Hash.assign(1, 0);
+  g->TreeBFS_ID.assign(1, g->head);
+  g->TreeBFS_adj.assign(1, g->head);
+  g->TreeBFS_pos.assign(1, 0);
+  g->TreeBFS_Hash.push_back(1);
+  std::queue<int> frontier;
+  for (int i = g->ChildHash[g->head].second; i < g->ChildHash[g->head].second + g->ChildHash[g->head].first; i++)
+    frontier.push(g->Childs[i]);
+  while (!frontier.empty()) {
+    size_t size = frontier.size();
+    for (size_t i = 0; i < size; i++) {
+      int id = frontier.front();
+      frontier.pop();
+      g->TreeBFS.push_back(id);
+      for (int64_t k = g->H2H_pos_hash[id]; k < g->H2H_pos_hash[id + 1]; k++) {
+        g->TreeBFS_ID.push_back(id);
+        g->TreeBFS_adj.push_back(g->H2H_pos_ID[k]);
+        g->TreeBFS_pos.push_back(g->H2H_pos_POS[k]);
+      }
+      for (int j = g->ChildHash[id].second; j < g->ChildHash[id].second + g->ChildHash[id].first; j++)
+        frontier.push(g->Childs[j]);
+    }
+    g->TreeHash.push_back((int)g->TreeBFS.size());
+    g->TreeBFS_Hash.push_back((int64_t)g->TreeBFS_ID.size());
+  }
+  g->t_bfs = usec_since(t0);
+}
+
+// ---------------------------------------------------------------------------
+// Label builds.  Per-node update rule of makeH2HLabel_noHub_serial
+// (H2HConsturction.cu:766-790), with the L4 father-chain loop the reference
+// keeps commented out (:783-789, live at tanslate.cu:378-385).
+// ---------------------------------------------------------------------------
+static inline void label_node(Oracle* g, int nodeID, bool l4) {
+  int* dis = g->H2H_dis.data();
+  for (int64_t temp = g->H2H_pos_hash[(int64_t)nodeID + 1] - 2; temp >= g->H2H_pos_hash[nodeID]; temp--) {
+    int adjID = g->H2H_pos_ID[temp];
+    int pos = g->H2H_pos_POS[temp];
+    int64_t indexNode = g->H2H_dis_hash[nodeID];
+    int64_t indexAdj = g->H2H_dis_hash[adjID];
+    int t = dis[indexNode + pos];
+    for (int j = pos - 1; j >= 0; j--)
+      dis[indexNode + j] = std::min(dis[indexNode + j], t + dis[indexAdj + j]);
+    if (l4) {
+      int64_t f = nodeID;
+      for (int j = (int)(g->H2H_dis_hash[nodeID + 1] - g->H2H_dis_hash[nodeID]) - 2; j > pos; j--) {
+        f = g->father[f];
+        dis[indexNode + j] = std::min(dis[indexNode + j], t + dis[g->H2H_dis_hash[f] + pos]);
+      }
+    }
+  }
+}
+
+// serial: H2HConsturction.cu:727-800
+static void labels_serial(Oracle* g, bool l4) {
+  for (size_t h = 0; h + 1 < g->TreeHash.size(); h++)
+    for (int i = g->TreeHash[h]; i < g->TreeHash[h + 1]; i++) label_node(g, g->TreeBFS[i], l4);
+}
+
+// PF on CPU threads: makeH2HLabel_noHub_multiThred, H2HConsturction.cu:806-878
+static void labels_pf(Oracle* g, bool l4, int threads) {
+  for (int64_t h = 0; h < (int64_t)g->TreeHash.size() - 1; h++) {
+    int64_t s = g->TreeHash[h], e = g->TreeHash[h + 1];
+    int64_t tn = std::min((int64_t)threads, e - s);
+    int64_t chunk = (e - s) / tn + 1;
+    std::vector<std::vector<int64_t>> cand(tn);
+    int64_t it = 0, temp = 0;
+    for (int64_t i = s; i < e; i++) {
+      cand[it].push_back(g->TreeBFS[i]);
+      if (++temp == chunk) { it++; temp = 0; }
+    }
+    if (tn == 1) {
+      for (auto v : cand[0]) label_node(g, (int)v, l4);
+      continue;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < tn; t++)
+      th.emplace_back([g, &cand, t, l4]() { for (auto v : cand[t]) label_node(g, (int)v, l4); });
+    for (auto& x : th) x.join();
+  }
+}
+
+static inline void atomic_min_int(int* p, int v) {
+  int cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+  while (v < cur && !__atomic_compare_exchange_n(p, &cur, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}
+}
+
+// LL, race-free: one task per (v, bag entry) of a level with an atomic min,
+// the per-entry rule of kernel makeH2HLabel_noCommunication_D_noHub_3
+// (kernel_functions.cu:717-744; L4 loop :738-743) over TreeBFS_Hash levels
+// (:776-792).  The reference's CPU LL (H2HConsturction.cu:881-959) declares
+// its mutex inside the loop and races; this keeps the rule, not the race.
+static void labels_ll(Oracle* g, bool l4, int threads) {
+  int* dis = g->H2H_dis.data();
+  for (int64_t h = 0; h < (int64_t)g->TreeBFS_Hash.size() - 1; h++) {
+    int64_t s = g->TreeBFS_Hash[h], e = g->TreeBFS_Hash[h + 1];
+    auto work = [g, dis, l4](int64_t a, int64_t b) {
+      for (int64_t k = a; k < b; k++) {
+        int nodeID = g->TreeBFS_ID[k];
+        int pos = g->TreeBFS_pos[k];
+        int64_t indexNode = g->H2H_dis_hash[nodeID];
+        int64_t indexAdj = g->H2H_dis_hash[g->TreeBFS_adj[k]];
+        int t = __atomic_load_n(&dis[indexNode + pos], __ATOMIC_RELAXED);
+        for (int64_t j = pos - 1; j >= 0; j--)
+          atomic_min_int(&dis[indexNode + j], t + __atomic_load_n(&dis[indexAdj + j], __ATOMIC_RELAXED));
+        if (l4) {
+          int f = nodeID;
+          for (int64_t j = (int)(g->H2H_dis_hash[nodeID + 1] - g->H2H_dis_hash[nodeID]) - 2; j > pos; j--) {
+            f = g->father[f];
+            atomic_min_int(&dis[indexNode + j], t + dis[g->H2H_dis_hash[f] + pos]);
+          }
+        }
+      }
+    };
+    int64_t tn = std::min((int64_t)threads, e - s);
+    if (tn <= 1) { work(s, e); continue; }
+    int64_t chunk = (e - s) / tn + 1;
+    std::vector<std::thread> th;
+    for (int64_t t = 0; t < tn; t++) {
+      int64_t a = s + t * chunk, b = std::min(e, a + chunk);
+      if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto& x : th) x.join();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Queries.  findLCA_noHub (H2HConsturction.cu:198-221),
+// H2HDistancQuery_UsingLCA_noHub (Query.cu:108-122),
+// H2HDistancQuery_noLCA_noHub (Query.cu:124-151),
+// H2Hquery_bunch_MultiThread_noHub (Query.cu:172-202).
+// ---------------------------------------------------------------------------
+static int find_lca(const Oracle* g, int x, int y) {
+  if (x == y) return x;
+  int a = g->firstAppeare[x], b = g->firstAppeare[y];
+  if (a > b) std::swap(a, b);
+  int k = (int)(std::log2(b - a + 1));
+  int id1 = g->RMQ_ID[g->RMQHash[a] + (int64_t)k];
+  int id2 = g->RMQ_ID[g->RMQHash[b - (1 << k) + 1] + (int64_t)k];
+  return g->H2H_TreeHeight_Hash[id1] <= g->H2H_TreeHeight_Hash[id2] ? id1 : id2;
+}
+
+static int query_lca(const Oracle* g, int x, int y) {
+  if (x == y) return 0;
+  int L = find_lca(g, x, y);
+  int r = INT_MAX;
+  for (int64_t i = g->H2H_pos_hash[L]; i < g->H2H_pos_hash[L + 1]; i++) {
+    int t = g->H2H_dis[g->H2H_dis_hash[x] + g->H2H_pos_POS[i]] + g->H2H_dis[g->H2H_dis_hash[y] + g->H2H_pos_POS[i]];
+    if (t < r) r = t;
+  }
+  return r;
+}
+
+static int query_nolca(const Oracle* g, int x, int y) {
+  if (x == y) return 0;
+  int A = g->firstAppeare[x], B = g->firstAppeare[y];
+  int a = (A < B) * A + (B <= A) * B;
+  int b = (A < B) * B + (B <= A) * A;
+  int k = (int)(std::log2((double)(b - a + 1)));
+  int id1 = g->RMQ_ID[g->RMQHash[a] + (int64_t)k];
+  int id2 = g->RMQ_ID[g->RMQHash[b - (1 << k) + 1] + (int64_t)k];
+  int sz = g->H2H_TreeHeight_Hash[id2];
+  if (g->H2H_TreeHeight_Hash[id1] <= g->H2H_TreeHeight_Hash[id2]) sz = g->H2H_TreeHeight_Hash[id1];
+  int r = INT_MAX;
+  const int* dx = g->H2H_dis.data() + g->H2H_dis_hash[x];
+  const int* dy = g->H2H_dis.data() + g->H2H_dis_hash[y];
+  for (int i = 0; i < sz; i++) {
+    int t = dx[i] + dy[i];
+    if (t < r) r = t;
+  }
+  return r;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI used by tests/ and bench.py (ctypes).  Status: 0 ok, <0 error.
+// ===========================================================================
+extern "C" {
+
+enum { ORC_CPU = 0, ORC_GPU_FAST = 1, ORC_GPU_LITERAL = 2 };
+
+void* orc_create(int n, long long m, const int32_t* arc_u, const int32_t* arc_v, const int32_t* arc_w,
+                 const int32_t* index_ids, int height_file, int height_arg, const int32_t* cut_lr,
+                 int threads) {
+  Oracle* g = new Oracle();
+  g->NodeNumber = n;
+  g->threadNumber = threads > 0 ? threads : 1;
+  g->NE_P_id.assign(index_ids, index_ids + n);
+  g->ID_hash.assign(n, -1);
+  for (int i = 0; i < n; i++) g->ID_hash[index_ids[i]] = i;
+  g->adjList.assign(n, {});
+  long long actual = m;
+  for (long long i = 0; i < m; i++) {
+    if (arc_u[i] == arc_v[i]) { actual--; continue; }
+    g->adjList[arc_u[i]].push_back(Pairs(arc_v[i], arc_w[i]));
+  }
+  g->EdgeNumber = actual;
+  int H = std::min(height_file, height_arg);
+  g->TreeHeight = H;
+  long long full = (1LL << H) - 1;
+  g->partition_Tree.assign(full, Pairs());
+  for (long long i = 0; i < full; i++) g->partition_Tree[i] = Pairs(cut_lr[2 * i], cut_lr[2 * i + 1]);
+  finish_load(g);
+  return g;
+}
+
+// File loader with the reference's parsing (Graph_Fundamental.cu:21-78).
+void* orc_load_files(const char* edge_file, const char* id_file, const char* cut_file, int height_arg,
+                     int threads) {
+  std::ifstream fid(id_file);
+  if (!fid) return nullptr;
+  int n = 0;
+  fid >> n;
+  std::vector<int32_t> ids(n, 0);
+  for (int i = 0; i < n; i++) {
+    int hash, id;
+    double la, gr;
+    fid >> hash >> la >> gr >> id;
+    if (hash >= 0 && hash < n) ids[hash] = id;
+  }
+  std::ifstream fe(edge_file);
+  if (!fe) return nullptr;
+  long long m = 0;
+  fe >> n >> m;
+  std::vector<int32_t> u(m), v(m), w(m);
+  for (long long i = 0; i < m; i++) {
+    char s;
+    fe >> s >> u[i] >> v[i] >> w[i];
+    u[i] -= 1;
+    v[i] -= 1;
+  }
+  std::ifstream fc(cut_file);
+  if (!fc) return nullptr;
+  int hf = 0;
+  fc >> hf;
+  int H = std::min(hf, height_arg);
+  long long full = (1LL << H) - 1;
+  std::vector<int32_t> lr(2 * full);
+  for (long long i = 0; i < full; i++) {
+    int h;
+    fc >> h >> lr[2 * i] >> lr[2 * i + 1];
+  }
+  return orc_create(n, m, u.data(), v.data(), w.data(), ids.data(), hf, height_arg, lr.data(), threads);
+}
+
+void orc_destroy(void* h) { delete (Oracle*)h; }
+const char* orc_error(void* h) { return ((Oracle*)h)->err.c_str(); }
+
+// Partition rank tree + contraction + tree structure.
+// mode ORC_CPU: Partition_CH_Construct_noHub (main.cu:126-147)
+// mode ORC_GPU_*: TDAndH2HConstruct_Tradition_plus_G_2_noHub (main.cu:84-124)
+int orc_contract(void* h, int mode, int change_height) {
+  Oracle* g = (Oracle*)h;
+  if (change_height < 1 || change_height > g->TreeHeight) return fail(g, "changeHeight must be in [1, H]");
+  g->changeHeight = change_height;
+  make_adjcent_node(g);
+  if ((long long)g->nonAdjcentNode.size() != g->NodeNumber) return fail(g, "partition heap does not cover every vertex");
+  g->tempnonAdjcentNodeIndex = INT_MAX;
+  auto t0 = Clock::now();
+  if (mode == ORC_CPU) {
+    partition_td_t(g, change_height);
+    g->t_contract_partition = usec_since(t0);
+    t0 = Clock::now();
+    chusing_heap_2(g, top_vertices(g, change_height));
+  } else {
+    if (g->TreeHeight != change_height) {
+      std::vector<Pairs> info = g->CHInfo_H;
+      std::vector<int> holes(g->NodeNumber, 0);
+      if (mode == ORC_GPU_FAST) {
+        g->CHInfo.assign(g->NodeNumber, Pairs());
+        g->CHChild.assign(g->NodeNumber, 0);
+        load_adjlist_into_map(g);
+      }
+      for (int hgt = g->TreeHeight; hgt > change_height; hgt--) {
+        long long lo = (1LL << (hgt - 1)) - 1, hi = (1LL << hgt) - 2;
+        for (long long p = lo; p <= hi; p++) {
+          if (mode == ORC_GPU_LITERAL) {
+            if (gpu_partition_literal(g, p, info) != 0) return -1;
+          } else {
+            gpu_partition_fast(g, p, holes);
+          }
+        }
+      }
+      if (mode == ORC_GPU_FAST) {
+        // hand the state back in the form translateInTD + uploadCHset see it
+        for (int v = 0; v < g->NodeNumber; v++) g->CHInfo_H[v].second = g->CHInfo[v].second;
+        std::vector<int> tops = top_vertices(g, change_height);
+        g->CHInfo.assign(g->NodeNumber, Pairs());
+        g->CHChild.assign(g->NodeNumber, 0);
+        for (int v : tops) g->CHInfo[v] = Pairs((int)g->CHAdjlist[v].size(), g->CHInfo_H[v].second);
+        for (int h2 = change_height; h2 > 0; h2--) {
+          long long lo = (1LL << (h2 - 1)) - 1, hi = (1LL << h2) - 2;
+          for (long long i = hi; i >= lo; i--)
+            g->tempnonAdjcentNodeIndex = std::min(g->NANHash[i].first, g->tempnonAdjcentNodeIndex);
+        }
+        g->t_contract_partition = usec_since(t0);
+        t0 = Clock::now();
+        chusing_heap_2(g, tops);
+        g->t_contract_top = usec_since(t0);
+        build_tree(g);
+        return 0;
+      }
+      g->CHInfo_H = info;
+    }
+    g->t_contract_partition = usec_since(t0);
+    t0 = Clock::now();
+    upload_chset(g, change_height);
+    chusing_heap_2(g, top_vertices(g, change_height));
+  }
+  g->t_contract_top = usec_since(t0);
+  build_tree(g);
+  return 0;
+}
+
+int orc_labels_alloc(void* h) {
+  malloc_labels((Oracle*)h);
+  return 0;
+}
+
+// method 0 serial, 1 PF threads, 2 LL threads (race-free).  Returns usec.
+long long orc_labels_build(void* h, int method, int l4, int threads) {
+  Oracle* g = (Oracle*)h;
+  auto t0 = Clock::now();
+  if (method == 0) labels_serial(g, l4 != 0);
+  else if (method == 1) labels_pf(g, l4 != 0, threads);
+  else labels_ll(g, l4 != 0, threads);
+  g->t_label = usec_since(t0);
+  return g->t_label;
+}
+
+// kind 0 = LCA + pos (UsingLCA), 1 = noLCA.  Returns usec.
+long long orc_query(void* h, int kind, const int32_t* x, const int32_t* y, int32_t* out, long long n, int threads) {
+  const Oracle* g = (const Oracle*)h;
+  auto t0 = Clock::now();
+  auto run = [g, kind, x, y, out](long long a, long long b) {
+    for (long long j = a; j < b; j++) out[j] = kind == 0 ? query_lca(g, x[j], y[j]) : query_nolca(g, x[j], y[j]);
+  };
+  if (threads <= 1) run(0, n);
+  else {
+    long long chunk = n / threads + 1;
+    std::vector<std::thread> th;
+    for (int i = 0; i < threads; i++) {
+      long long a = (long long)i * chunk, b = std::min(n, (long long)(i + 1) * chunk);
+      if (a < b) th.emplace_back(run, a, b);
+    }
+    for (auto& t : th) t.join();
+  }
+  return usec_since(t0);
+}
+
+int orc_lca(void* h, int x, int y) { return find_lca((Oracle*)h, x, y); }
+
+// ---- sizes and getters ----
+long long orc_size(void* h, int what) {
+  Oracle* g = (Oracle*)h;
+  switch (what) {
+    case 0: return g->NodeNumber;
+    case 1: return g->H2H_Size;
+    case 2: return (long long)g->H2H_pos_ID.size();
+    case 3: return (long long)g->OULA_DFS_.size();
+    case 4: return (long long)g->RMQ_ID.size();
+    case 5: return (long long)g->TreeHash.size() - 1;
+    case 6: return g->CHTreeHeight;
+    case 7: return g->head;
+    case 8: return g->TreeHeight;
+    case 9: return g->lubLength;
+    case 10: return g->vertexesConstructedInGPU;
+    case 11: {
+      long long s = 0;
+      for (auto& m : g->CHAdjlist) s += (long long)m.size();
+      return s;
+    }
+    case 12: return (long long)g->TreeBFS_ID.size();
+    case 13: return g->EdgeNumber;
+  }
+  return -1;
+}
+
+// Timings in usec: 0 partition tree, 1 partition contraction, 2 top phase,
+// 3 tree, 4 label malloc, 5 BFS, 6 label build.
+long long orc_time(void* h, int what) {
+  Oracle* g = (Oracle*)h;
+  const long long t[] = {g->t_partition, g->t_contract_partition, g->t_contract_top, g->t_tree,
+                         g->t_malloc, g->t_bfs, g->t_label};
+  return (what >= 0 && what < 7) ? t[what] : -1;
+}
+
+int orc_get_i32(void* h, int what, int32_t* out) {
+  Oracle* g = (Oracle*)h;
+  const int N = g->NodeNumber;
+  switch (what) {
+    case 0: for (int i = 0; i < N; i++) out[i] = g->nonAdjcentNode[i].second; return 0;   // elimination order
+    case 1: std::copy(g->father.begin(), g->father.end(), out); return 0;
+    case 2: std::copy(g->H2H_TreeHeight_Hash.begin(), g->H2H_TreeHeight_Hash.end(), out); return 0;  // depth
+    case 3: std::copy(g->firstAppeare.begin(), g->firstAppeare.end(), out); return 0;
+    case 4: std::copy(g->H2H_dis.begin(), g->H2H_dis.end(), out); return 0;
+    case 5: std::copy(g->H2H_pos_ID.begin(), g->H2H_pos_ID.end(), out); return 0;
+    case 6: std::copy(g->H2H_pos_POS.begin(), g->H2H_pos_POS.end(), out); return 0;
+    case 7: for (size_t i = 0; i < g->OULA_DFS_.size(); i++) out[i] = g->OULA_DFS_[i].first; return 0;
+    case 8: std::copy(g->RMQ_ID.begin(), g->RMQ_ID.end(), out); return 0;
+    case 9: std::copy(g->TreeBFS.begin(), g->TreeBFS.end(), out); return 0;
+    case 10: std::copy(g->TreeHash.begin(), g->TreeHash.end(), out); return 0;
+    case 11: for (int i = 0; i < N; i++) out[i] = g->nonAdjcentNode[i].third; return 0;  // recorded height
+    case 12: for (int i = 0; i < N; i++) out[i] = g->nonAdjcentNode[i].first; return 0;  // recorded degree
+    case 13: for (size_t i = 0; i < g->NANHash.size(); i++) { out[3 * i] = g->NANHash[i].first; out[3 * i + 1] = g->NANHash[i].second; out[3 * i + 2] = g->NANHash[i].NodeID; } return 0;
+    case 14: std::copy(g->TreeBFS_ID.begin(), g->TreeBFS_ID.end(), out); return 0;
+    case 15: std::copy(g->TreeBFS_adj.begin(), g->TreeBFS_adj.end(), out); return 0;
+    case 16: std::copy(g->TreeBFS_pos.begin(), g->TreeBFS_pos.end(), out); return 0;
+  }
+  return -1;
+}
+
+int orc_get_i64(void* h, int what, int64_t* out) {
+  Oracle* g = (Oracle*)h;
+  switch (what) {
+    case 0: std::copy(g->H2H_dis_hash.begin(), g->H2H_dis_hash.end(), out); return 0;
+    case 1: std::copy(g->H2H_pos_hash.begin(), g->H2H_pos_hash.end(), out); return 0;
+    case 2: std::copy(g->RMQHash.begin(), g->RMQHash.end(), out); return 0;
+    case 3: std::copy(g->TreeBFS_Hash.begin(), g->TreeBFS_Hash.end(), out); return 0;
+  }
+  return -1;
+}
+
+// Bags (CHAdjlist after contraction), as CSR sorted by neighbour id:
+// off[N+1], nbr[], w[].
+int orc_get_bags(void* h, int64_t* off, int32_t* nbr, int32_t* w) {
+  Oracle* g = (Oracle*)h;
+  int64_t k = 0;
+  for (int v = 0; v < g->NodeNumber; v++) {
+    off[v] = k;
+    std::vector<std::pair<int, int>> e;
+    for (auto& it : g->CHAdjlist[v]) e.emplace_back(it.first, it.second.first);
+    std::sort(e.begin(), e.end());
+    for (auto& p : e) { nbr[k] = p.first; w[k] = p.second; k++; }
+  }
+  off[g->NodeNumber] = k;
+  return 0;
+}
+
+int orc_get_layer_fraction(void* h, double* out) {
+  Oracle* g = (Oracle*)h;
+  std::copy(g->layerFraction.begin(), g->layerFraction.end(), out);
+  return 0;
+}
+
+// Dijkstra from one source over the loaded arcs (pins labels and queries).
+int orc_dijkstra(void* h, int src, int64_t* out) {
+  Oracle* g = (Oracle*)h;
+  const int N = g->NodeNumber;
+  std::vector<int64_t> d(N, INT64_MAX);
+  using QE = std::pair<int64_t, int>;
+  std::priority_queue<QE, std::vector<QE>, std::greater<QE>> pq;
+  d[src] = 0;
+  pq.emplace(0, src);
+  while (!pq.empty()) {
+    auto [du, u] = pq.top();
+    pq.pop();
+    if (du != d[u]) continue;
+    for (auto& e : g->adjList[u]) {
+      int64_t nd = du + e.second;
+      if (nd < d[e.first]) { d[e.first] = nd; pq.emplace(nd, e.first); }
+    }
+  }
+  std::copy(d.begin(), d.end(), out);
+  return 0;
+}
+
+}  // extern "C"

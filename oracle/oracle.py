@@ -1,0 +1,199 @@
+"""ctypes front end of the CPU oracle (oracle/h2h_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product path never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libh2h_oracle.so")
+
+CPU, GPU_FAST, GPU_LITERAL = 0, 1, 2
+Q_LCA, Q_NOLCA = 0, 1
+SERIAL, PF, LL = 0, 1, 2
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_longlong, i32p, i32p, i32p, i32p,
+                                 ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int]
+        L.orc_load_files.restype = P
+        L.orc_load_files.argtypes = [ctypes.c_char_p] * 3 + [ctypes.c_int, ctypes.c_int]
+        L.orc_destroy.argtypes = [P]
+        L.orc_error.restype = ctypes.c_char_p
+        L.orc_error.argtypes = [P]
+        L.orc_contract.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_labels_alloc.argtypes = [P]
+        L.orc_labels_build.restype = ctypes.c_longlong
+        L.orc_labels_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_query.restype = ctypes.c_longlong
+        L.orc_query.argtypes = [P, ctypes.c_int, i32p, i32p, i32p, ctypes.c_longlong, ctypes.c_int]
+        L.orc_lca.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.orc_size.restype = ctypes.c_longlong
+        L.orc_size.argtypes = [P, ctypes.c_int]
+        L.orc_time.restype = ctypes.c_longlong
+        L.orc_time.argtypes = [P, ctypes.c_int]
+        L.orc_get_i32.argtypes = [P, ctypes.c_int, i32p]
+        L.orc_get_i64.argtypes = [P, ctypes.c_int, i64p]
+        L.orc_get_bags.argtypes = [P, i64p, i32p, i32p]
+        L.orc_get_layer_fraction.argtypes = [P, f64p]
+        L.orc_dijkstra.argtypes = [P, ctypes.c_int, i64p]
+        _lib = L
+    return _lib
+
+
+class OracleIndex:
+    """The reference pipeline on the CPU: load -> contract -> tree -> labels -> queries."""
+
+    def __init__(self, n, arc_u, arc_v, arc_w, index_ids, cut_lr, height_file, height_arg=None,
+                 threads=1):
+        L = lib()
+        self.n = int(n)
+        arc_u = np.ascontiguousarray(arc_u, np.int32)
+        arc_v = np.ascontiguousarray(arc_v, np.int32)
+        arc_w = np.ascontiguousarray(arc_w, np.int32)
+        index_ids = np.ascontiguousarray(index_ids, np.int32)
+        cut = np.ascontiguousarray(np.asarray(cut_lr, np.int32).reshape(-1))
+        if height_arg is None:
+            height_arg = height_file
+        self.threads = threads
+        self.h = L.orc_create(self.n, len(arc_u), arc_u, arc_v, arc_w, index_ids, int(height_file),
+                              int(height_arg), cut, int(threads))
+
+    @classmethod
+    def from_files(cls, edge, idf, cut, height, threads=1):
+        obj = cls.__new__(cls)
+        L = lib()
+        obj.h = L.orc_load_files(edge.encode(), idf.encode(), cut.encode(), int(height), int(threads))
+        if not obj.h:
+            raise IOError("oracle could not read input files")
+        obj.n = int(L.orc_size(obj.h, 0))
+        obj.threads = threads
+        return obj
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RuntimeError(lib().orc_error(self.h).decode())
+
+    def contract(self, mode, change_height):
+        self._check(lib().orc_contract(self.h, int(mode), int(change_height)))
+
+    def labels(self, method=PF, l4=True, threads=None):
+        L = lib()
+        self._check(L.orc_labels_alloc(self.h))
+        return L.orc_labels_build(self.h, int(method), int(bool(l4)), int(threads or self.threads))
+
+    def query(self, kind, x, y, threads=1):
+        x = np.ascontiguousarray(x, np.int32)
+        y = np.ascontiguousarray(y, np.int32)
+        out = np.empty(len(x), np.int32)
+        us = lib().orc_query(self.h, int(kind), x, y, out, len(x), int(threads))
+        return out, us
+
+    def size(self, what):
+        return int(lib().orc_size(self.h, what))
+
+    def time_us(self, what):
+        return int(lib().orc_time(self.h, what))
+
+    def _i32(self, what, n):
+        out = np.empty(n, np.int32)
+        self._check(lib().orc_get_i32(self.h, what, out))
+        return out
+
+    def _i64(self, what, n):
+        out = np.empty(n, np.int64)
+        self._check(lib().orc_get_i64(self.h, what, out))
+        return out
+
+    # names follow the reference's members
+    def elimination_order(self):
+        return self._i32(0, self.n)
+
+    def recorded_height(self):
+        return self._i32(11, self.n)
+
+    def father(self):
+        return self._i32(1, self.n)
+
+    def depth(self):
+        return self._i32(2, self.n)
+
+    def first_appear(self):
+        return self._i32(3, self.n)
+
+    def euler(self):
+        return self._i32(7, self.size(3))
+
+    def dis(self):
+        return self._i32(4, self.size(1))
+
+    def dis_hash(self):
+        return self._i64(0, self.n + 1)
+
+    def pos_hash(self):
+        return self._i64(1, self.n + 1)
+
+    def pos_id(self):
+        return self._i32(5, self.size(2))
+
+    def pos_pos(self):
+        return self._i32(6, self.size(2))
+
+    def tree_bfs(self):
+        return self._i32(9, self.n)
+
+    def tree_hash(self):
+        return self._i32(10, self.size(5) + 1)
+
+    def nan_hash(self):
+        h = self.size(8)
+        return self._i32(13, 3 * ((1 << h) - 1)).reshape(-1, 3)
+
+    def bags(self):
+        m = self.size(11)
+        off = np.empty(self.n + 1, np.int64)
+        nbr = np.empty(max(m, 1), np.int32)
+        w = np.empty(max(m, 1), np.int32)
+        self._check(lib().orc_get_bags(self.h, off, nbr, w))
+        return off, nbr[:m], w[:m]
+
+    def layer_fraction(self):
+        out = np.empty(self.size(8), np.float64)
+        lib().orc_get_layer_fraction(self.h, out)
+        return out
+
+    def dijkstra(self, src):
+        out = np.empty(self.n, np.int64)
+        lib().orc_dijkstra(self.h, int(src), out)
+        return out
