@@ -1,0 +1,515 @@
+// extern "C" entry points of libh2h (declared in include/h2h.h).
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <string>
+
+#include "h2h_internal.h"
+
+using namespace h2h;
+
+namespace {
+
+// Whitespace-separated integer/float reader over a whole file (fast path for
+// the multi-million-line Edge files).
+struct TextReader {
+  std::string buf;
+  size_t pos = 0;
+  bool open(const char* path) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    buf.resize(sz > 0 ? (size_t)sz : 0);
+    size_t got = sz > 0 ? std::fread(&buf[0], 1, (size_t)sz, f) : 0;
+    std::fclose(f);
+    buf.resize(got);
+    return true;
+  }
+  void skip() {
+    while (pos < buf.size() && (buf[pos] == ' ' || buf[pos] == '\t' || buf[pos] == '\n' || buf[pos] == '\r')) pos++;
+  }
+  bool token(std::string& out) {
+    skip();
+    if (pos >= buf.size()) return false;
+    size_t s = pos;
+    while (pos < buf.size() && !(buf[pos] == ' ' || buf[pos] == '\t' || buf[pos] == '\n' || buf[pos] == '\r')) pos++;
+    out.assign(buf, s, pos - s);
+    return true;
+  }
+  bool i64(long long& v) {
+    skip();
+    if (pos >= buf.size()) return false;
+    bool neg = false;
+    if (buf[pos] == '-' || buf[pos] == '+') { neg = buf[pos] == '-'; pos++; }
+    if (pos >= buf.size() || buf[pos] < '0' || buf[pos] > '9') return false;
+    long long x = 0;
+    while (pos < buf.size() && buf[pos] >= '0' && buf[pos] <= '9') x = x * 10 + (buf[pos++] - '0');
+    v = neg ? -x : x;
+    return true;
+  }
+  bool any() {  // skip one token of any kind (coordinates)
+    std::string t;
+    return token(t);
+  }
+};
+
+h2h_status need_ctx(h2h_ctx* ctx) { return ctx ? H2H_OK : H2H_ERR_ARG; }
+h2h_status need_gpu(h2h_ctx* ctx) {
+  return ctx->device >= 0 ? H2H_OK : set_err(ctx, H2H_ERR_NOGPU, "host-only context (created with device -1)");
+}
+
+}  // namespace
+
+extern "C" {
+
+h2h_status h2h_create(h2h_ctx** out, int device) {
+  if (!out) return H2H_ERR_ARG;
+  *out = nullptr;
+  if (device == -1) {  // host-only context: input loading and validation, no device work
+    h2h_ctx* ctx = new h2h_ctx();
+    ctx->device = -1;
+    *out = ctx;
+    return H2H_OK;
+  }
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return H2H_ERR_NOGPU;
+  if (device < 0 || device >= count) return H2H_ERR_ARG;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return H2H_ERR_NOGPU;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return H2H_ERR_NOGPU;
+  if (hipSetDevice(device) != hipSuccess) return H2H_ERR_NOGPU;
+  h2h_ctx* ctx = new h2h_ctx();
+  ctx->device = device;
+  ctx->num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return H2H_ERR_HIP;
+  }
+  *out = ctx;
+  return H2H_OK;
+}
+
+void h2h_destroy(h2h_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->device >= 0) (void)hipSetDevice(ctx->device);
+  delete ctx;
+}
+
+const char* h2h_last_error(const h2h_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+h2h_status h2h_set_graph(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_u, const int32_t* arc_v,
+                         const int32_t* arc_w, const int32_t* index_ids, int height_file, int height_arg,
+                         const int32_t* cut_lr) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (n < 1 || m < 0 || (m > 0 && (!arc_u || !arc_v || !arc_w)) || !index_ids || !cut_lr)
+    return set_err(ctx, H2H_ERR_ARG, "h2h_set_graph: bad sizes or null arrays");
+  const int H = std::min(height_file, height_arg);
+  if (H < 1 || H > 30) return set_err(ctx, H2H_ERR_ARG, "partition height must be in [1, 30]");
+  ctx->graph_loaded = ctx->contracted = ctx->tree_ready = ctx->labels_ready = false;
+  // arcs: self-loops dropped, kept directed as listed (Graph_Fundamental.cu:46-55)
+  std::vector<int64_t> deg(n + 1, 0);
+  int64_t kept = 0;
+  for (int64_t i = 0; i < m; i++) {
+    const int32_t u = arc_u[i], v = arc_v[i];
+    if (u < 0 || u >= n || v < 0 || v >= n) return set_err(ctx, H2H_ERR_INPUT, "arc endpoint out of range");
+    if (arc_w[i] < 0) return set_err(ctx, H2H_ERR_INPUT, "negative arc weight");
+    if (u == v) continue;
+    deg[u + 1]++;
+    kept++;
+  }
+  for (int v = 0; v < n; v++) deg[v + 1] += deg[v];
+  ctx->adj_off = deg;
+  ctx->adj_nbr.assign(kept, 0);
+  ctx->adj_w.assign(kept, 0);
+  {
+    std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
+    for (int64_t i = 0; i < m; i++) {
+      const int32_t u = arc_u[i], v = arc_v[i];
+      if (u == v) continue;
+      ctx->adj_nbr[fill[u]] = v;
+      ctx->adj_w[fill[u]++] = arc_w[i];
+    }
+  }
+  // simple, symmetric: no parallel arcs, every arc has its reverse
+  {
+    std::vector<int32_t> srt(ctx->adj_nbr);
+    for (int v = 0; v < n; v++) {
+      auto b = srt.begin() + ctx->adj_off[v], e = srt.begin() + ctx->adj_off[v + 1];
+      std::sort(b, e);
+      if (std::adjacent_find(b, e) != e) return set_err(ctx, H2H_ERR_INPUT, "parallel arcs between one vertex pair");
+    }
+    for (int v = 0; v < n; v++)
+      for (int64_t k = ctx->adj_off[v]; k < ctx->adj_off[v + 1]; k++) {
+        const int32_t u = srt[k];
+        if (!std::binary_search(srt.begin() + ctx->adj_off[u], srt.begin() + ctx->adj_off[u + 1], v))
+          return set_err(ctx, H2H_ERR_INPUT, "arc without its reverse arc");
+      }
+  }
+  // connectivity (the reference prints "disconnect!" and continues, Graph_Fundamental.cu:125-127)
+  {
+    std::vector<char> seen(n, 0);
+    std::vector<int32_t> q;
+    q.reserve(n);
+    q.push_back(0);
+    seen[0] = 1;
+    for (size_t i = 0; i < q.size(); i++)
+      for (int64_t k = ctx->adj_off[q[i]]; k < ctx->adj_off[q[i] + 1]; k++)
+        if (!seen[ctx->adj_nbr[k]]) {
+          seen[ctx->adj_nbr[k]] = 1;
+          q.push_back(ctx->adj_nbr[k]);
+        }
+    if ((int32_t)q.size() != n) return set_err(ctx, H2H_ERR_INPUT, "graph is disconnected");
+  }
+  ctx->index_ids.assign(index_ids, index_ids + n);
+  ctx->id_hash.assign(n, -1);
+  for (int i = 0; i < n; i++) {
+    const int32_t v = index_ids[i];
+    if (v < 0 || v >= n || ctx->id_hash[v] != -1) return set_err(ctx, H2H_ERR_INPUT, "ID array is not a permutation");
+    ctx->id_hash[v] = i;
+  }
+  const int64_t heapN = (1LL << H) - 1;
+  ctx->cut_lr.assign(cut_lr, cut_lr + 2 * heapN);
+  for (int64_t i = 0; i < heapN; i++) {
+    const int32_t l = ctx->cut_lr[2 * i], r = ctx->cut_lr[2 * i + 1];
+    if (l < 0 || r > n || l > r) return set_err(ctx, H2H_ERR_INPUT, "cut range outside [0, N)");
+  }
+  ctx->n = n;
+  ctx->arcs = kept;
+  ctx->H = H;
+  ctx->graph_loaded = true;
+  return H2H_OK;
+}
+
+h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_file, const char* cut_file,
+                          int tree_height) {
+  if (need_ctx(ctx) || !edge_file || !id_file || !cut_file) return H2H_ERR_ARG;
+  TextReader rid, re, rc;
+  if (!rid.open(id_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + id_file);
+  if (!re.open(edge_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + edge_file);
+  if (!rc.open(cut_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + cut_file);
+  long long n = 0;
+  if (!rid.i64(n) || n < 1 || n > INT32_MAX) return set_err(ctx, H2H_ERR_IO, "ID file: bad vertex count");
+  std::vector<int32_t> ids(n, -1);
+  for (long long i = 0; i < n; i++) {
+    long long idx, id;
+    if (!rid.i64(idx) || !rid.any() || !rid.any() || !rid.i64(id) || idx < 0 || idx >= n)
+      return set_err(ctx, H2H_ERR_IO, "ID file: malformed line");
+    ids[idx] = (int32_t)id;
+  }
+  long long en = 0, m = 0;
+  if (!re.i64(en) || !re.i64(m) || m < 0) return set_err(ctx, H2H_ERR_IO, "Edge file: bad header");
+  if (en != n) return set_err(ctx, H2H_ERR_IO, "Edge and ID files disagree on the vertex count");
+  std::vector<int32_t> u(m), v(m), w(m);
+  std::string tag;
+  for (long long i = 0; i < m; i++) {
+    long long a, b, c;
+    if (!re.token(tag) || !re.i64(a) || !re.i64(b) || !re.i64(c)) return set_err(ctx, H2H_ERR_IO, "Edge file: truncated");
+    u[i] = (int32_t)(a - 1);
+    v[i] = (int32_t)(b - 1);
+    w[i] = (int32_t)c;
+  }
+  long long hf = 0;
+  if (!rc.i64(hf) || hf < 1 || hf > 30) return set_err(ctx, H2H_ERR_IO, "Cut file: bad height");
+  const int H = std::min<int>((int)hf, tree_height);
+  if (H < 1) return set_err(ctx, H2H_ERR_ARG, "partition height must be >= 1");
+  const int64_t heapN = (1LL << H) - 1;
+  std::vector<int32_t> lr(2 * heapN);
+  for (int64_t i = 0; i < heapN; i++) {
+    long long h, l, r;
+    if (!rc.i64(h) || !rc.i64(l) || !rc.i64(r)) return set_err(ctx, H2H_ERR_IO, "Cut file: truncated");
+    lr[2 * i] = (int32_t)l;
+    lr[2 * i + 1] = (int32_t)r;
+  }
+  return h2h_set_graph(ctx, (int32_t)n, m, u.data(), v.data(), w.data(), ids.data(), (int)hf, tree_height, lr.data());
+}
+
+h2h_status h2h_partition_rank_tree(h2h_ctx* ctx, int32_t* nan_hash, int32_t* elim, double* layer_fraction) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (!ctx->graph_loaded) return set_err(ctx, H2H_ERR_STATE, "no graph loaded");
+  h2h_status st = build_partition_plans(ctx, 1);
+  ctx->plans.clear();
+  if (st) return st;
+  const int64_t heapN = (1LL << ctx->H) - 1;
+  if (nan_hash)
+    for (int64_t i = 0; i < heapN; i++) {
+      nan_hash[3 * i] = ctx->nan_start[i];
+      nan_hash[3 * i + 1] = ctx->nan_max[i];
+      nan_hash[3 * i + 2] = ctx->nan_count[i];
+    }
+  if (elim) std::copy(ctx->elim.begin(), ctx->elim.end(), elim);
+  if (layer_fraction) std::copy(ctx->layer_fraction.begin(), ctx->layer_fraction.end(), layer_fraction);
+  ctx->contracted = ctx->tree_ready = ctx->labels_ready = false;
+  return H2H_OK;
+}
+
+h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int thread_num, double* usec) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (!ctx->graph_loaded) return set_err(ctx, H2H_ERR_STATE, "no graph loaded");
+  if (change_height < 1 || change_height > ctx->H)
+    return set_err(ctx, H2H_ERR_ARG, "ChangeToGlobalHeight must be in [1, PartitionHeight]");
+  if (contract_order != H2H_CONTRACT_CPU_ORDER && contract_order != H2H_CONTRACT_GPU_ORDER)
+    return set_err(ctx, H2H_ERR_ARG, "contract_order must be 0 (CPU order) or 1 (GPU order)");
+  (void)hipSetDevice(ctx->device);
+  ctx->contracted = ctx->tree_ready = ctx->labels_ready = false;
+  ctx->change_height = change_height;
+  ctx->contract_order = contract_order;
+  h2h_status st = build_partition_plans(ctx, change_height);
+  if (st) return st;
+  st = run_contraction(ctx, usec);
+  if (st) return st;
+  ctx->plans.clear();
+  ctx->plans.shrink_to_fit();
+  if (contract_order == H2H_CONTRACT_CPU_ORDER) merge_cpu_order(ctx, thread_num);
+  st = build_tree_and_layout(ctx);
+  if (st) return st;
+  st = upload_index(ctx);
+  if (st) return st;
+  ctx->contracted = true;
+  return H2H_OK;
+}
+
+h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* usec) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (!ctx->tree_ready) return set_err(ctx, H2H_ERR_STATE, "contract (or upload an index) first");
+  if (label_kind != H2H_LABEL_L3 && label_kind != H2H_LABEL_L4) return set_err(ctx, H2H_ERR_ARG, "label kind must be 3 or 4");
+  if (method != H2H_METHOD_PF && method != H2H_METHOD_LL && method != H2H_METHOD_BATCHED)
+    return set_err(ctx, H2H_ERR_ARG, "method must be PF (1), LL (2) or BATCHED (3)");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before rebuilding labels");
+  (void)hipSetDevice(ctx->device);
+  return run_label_build(ctx, label_kind, method, usec);
+}
+
+h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, const int32_t* dis,
+                             const int64_t* pos_hash, const int32_t* pos_id, const int32_t* pos_pos,
+                             const int32_t* father, const int32_t* depth) {
+  if (need_ctx(ctx) || n < 1 || !dis_hash || !dis || !pos_hash || !pos_id || !pos_pos || !father || !depth)
+    return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the index");
+  (void)hipSetDevice(ctx->device);
+  ctx->tree_ready = ctx->labels_ready = ctx->contracted = false;
+  ctx->n = n;
+  // bags from pos (self excluded), weights from the initial dis rows
+  ctx->bag_off.assign(n + 1, 0);
+  for (int v = 0; v < n; v++) {
+    const int64_t cnt = pos_hash[v + 1] - pos_hash[v] - 1;
+    if (cnt < 0 || dis_hash[v + 1] - dis_hash[v] != depth[v] || depth[v] < 1)
+      return set_err(ctx, H2H_ERR_INPUT, "inconsistent dis_hash / pos_hash / depth");
+    ctx->bag_off[v + 1] = ctx->bag_off[v] + cnt;
+  }
+  ctx->bag_nbr.assign(ctx->bag_off[n], 0);
+  ctx->bag_w.assign(ctx->bag_off[n], 0);
+  int head = -1;
+  for (int v = 0; v < n; v++) {
+    if (father[v] < 0) {
+      if (head != -1) return set_err(ctx, H2H_ERR_INPUT, "more than one root");
+      head = v;
+    } else if (father[v] >= n || depth[father[v]] + 1 != depth[v]) {
+      return set_err(ctx, H2H_ERR_INPUT, "father/depth inconsistent");
+    }
+    int64_t k = ctx->bag_off[v];
+    for (int64_t e = pos_hash[v]; e < pos_hash[v + 1] - 1; e++) {
+      const int32_t u = pos_id[e];
+      if (u < 0 || u >= n || pos_pos[e] != depth[u] - 1 || pos_pos[e] >= depth[v] - 1)
+        return set_err(ctx, H2H_ERR_INPUT, "bag entry is not a proper ancestor position");
+      ctx->bag_nbr[k] = u;
+      ctx->bag_w[k] = dis[dis_hash[v] + pos_pos[e]];
+      k++;
+    }
+    if (pos_id[pos_hash[v + 1] - 1] != v) return set_err(ctx, H2H_ERR_INPUT, "pos list must end with the vertex itself");
+  }
+  if (head < 0) return set_err(ctx, H2H_ERR_INPUT, "no root");
+  // a synthetic elimination record whose rank order reproduces `father`:
+  // deeper vertices first, ties by id; father = min-rank bag member needs the
+  // father to be the deepest bag member, which holds for elimination trees
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    return depth[a] != depth[b] ? depth[a] > depth[b] : a < b;
+  });
+  ctx->elim = order;
+  ctx->rec_deg.assign(n, 0);
+  ctx->rec_height.assign(n, 0);
+  h2h_status st = build_tree_and_layout(ctx);
+  if (st) return st;
+  for (int v = 0; v < n; v++)
+    if (ctx->father[v] != father[v] || ctx->depth[v] != depth[v])
+      return set_err(ctx, H2H_ERR_INPUT, "father is not the deepest bag member (not an elimination tree)");
+  return upload_index(ctx);
+}
+
+h2h_status h2h_query_device(h2h_ctx* ctx, int kind, int exec, const int32_t* d_x, const int32_t* d_y,
+                            int32_t* d_out, int64_t n, void* stream) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (!ctx->labels_ready) return set_err(ctx, H2H_ERR_STATE, "build labels first");
+  if (kind != H2H_QUERY_LCA && kind != H2H_QUERY_NOLCA) return set_err(ctx, H2H_ERR_ARG, "bad query kind");
+  if (exec != H2H_EXEC_PLAIN && exec != H2H_EXEC_BATCHED) return set_err(ctx, H2H_ERR_ARG, "bad exec mode");
+  if (n < 0 || (n > 0 && (!d_x || !d_y || !d_out))) return set_err(ctx, H2H_ERR_ARG, "bad query buffers");
+  if (n == 0) return H2H_OK;
+  hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+  if (exec == H2H_EXEC_BATCHED) {
+    h2h_status st = query_workspace(ctx, ctx->qws, n);
+    if (st) return st;
+  }
+  return launch_query(ctx, kind, exec, d_x, d_y, d_out, n, s, &ctx->qws);
+}
+
+h2h_status h2h_query(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const int32_t* y, int32_t* out,
+                     int64_t n, double* usec) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (!ctx->labels_ready) return set_err(ctx, H2H_ERR_STATE, "build labels first");
+  if (n < 0 || (n > 0 && (!x || !y || !out))) return set_err(ctx, H2H_ERR_ARG, "bad query buffers");
+  if (n == 0) {
+    if (usec) *usec = 0;
+    return H2H_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  for (int64_t i = 0; i < n; i++)
+    if (x[i] < 0 || x[i] >= ctx->n || y[i] < 0 || y[i] >= ctx->n)
+      return set_err(ctx, H2H_ERR_ARG, "query vertex id out of range");
+  h2h_status st;
+  if ((int64_t)ctx->q_cap < n) {
+    if ((st = dev_alloc(ctx, ctx->d_qx, n * sizeof(int32_t)))) return st;
+    if ((st = dev_alloc(ctx, ctx->d_qy, n * sizeof(int32_t)))) return st;
+    if ((st = dev_alloc(ctx, ctx->d_qout, n * sizeof(int32_t)))) return st;
+    ctx->q_cap = n;
+  }
+  hipStream_t s = ctx->stream;
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->d_qx.p, x, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->d_qy.p, y, n * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  hipEvent_t e0, e1;
+  H2H_HIP(ctx, hipEventCreate(&e0));
+  H2H_HIP(ctx, hipEventCreate(&e1));
+  H2H_HIP(ctx, hipEventRecord(e0, s));
+  st = h2h_query_device(ctx, kind, exec, ctx->d_qx.as<int32_t>(), ctx->d_qy.as<int32_t>(), ctx->d_qout.as<int32_t>(), n, s);
+  if (st) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return st;
+  }
+  H2H_HIP(ctx, hipEventRecord(e1, s));
+  H2H_HIP(ctx, hipMemcpyAsync(out, ctx->d_qout.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipStreamSynchronize(s));
+  float ms = 0;
+  H2H_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (usec) *usec = ms * 1000.0;
+  return H2H_OK;
+}
+
+h2h_status h2h_get_info(const h2h_ctx* ctx, h2h_info* o) {
+  if (!ctx || !o) return H2H_ERR_ARG;
+  std::memset(o, 0, sizeof(*o));
+  o->n = ctx->n;
+  o->arcs = ctx->arcs;
+  o->partition_height = ctx->H;
+  o->change_height = ctx->change_height;
+  o->label_entries = ctx->label_entries;
+  o->pos_entries = ctx->pos_hash.empty() ? 0 : ctx->pos_hash.back();
+  o->tree_height = ctx->tree_height;
+  o->levels = ctx->level_off.empty() ? 0 : (int64_t)ctx->level_off.size() - 1;
+  o->top_vertices = ctx->top_vertices;
+  o->head = ctx->head;
+  o->lub_entries = ctx->lub_entries;
+  const DevArray* arrs[] = {&ctx->d_dis_hash, &ctx->d_dis, &ctx->d_pos_hash, &ctx->d_pos_pos, &ctx->d_pos_w,
+                            &ctx->d_pos_rowoff, &ctx->d_depth, &ctx->d_bfs, &ctx->d_father, &ctx->d_anc,
+                            &ctx->d_lca, &ctx->d_tin};
+  for (auto* a : arrs) o->device_bytes += (int64_t)a->bytes;
+  o->t_partition_us = ctx->t_partition_us;
+  o->t_contract_us = ctx->t_contract_us;
+  o->t_tree_us = ctx->t_tree_us;
+  o->t_label_us = ctx->t_label_us;
+  return H2H_OK;
+}
+
+h2h_status h2h_get_order(const h2h_ctx* ctx, int32_t* elim, int32_t* rec_degree, int32_t* rec_height) {
+  if (!ctx) return H2H_ERR_ARG;
+  if (!ctx->tree_ready) return H2H_ERR_STATE;
+  if (elim) std::copy(ctx->elim.begin(), ctx->elim.end(), elim);
+  if (rec_degree) std::copy(ctx->rec_deg.begin(), ctx->rec_deg.end(), rec_degree);
+  if (rec_height) std::copy(ctx->rec_height.begin(), ctx->rec_height.end(), rec_height);
+  return H2H_OK;
+}
+
+h2h_status h2h_get_tree(const h2h_ctx* ctx, int32_t* father, int32_t* depth, int32_t* first_appear, int32_t* bfs,
+                        int32_t* level_off) {
+  if (!ctx) return H2H_ERR_ARG;
+  if (!ctx->tree_ready) return H2H_ERR_STATE;
+  if (father) std::copy(ctx->father.begin(), ctx->father.end(), father);
+  if (depth) std::copy(ctx->depth.begin(), ctx->depth.end(), depth);
+  if (first_appear) std::copy(ctx->first_appear.begin(), ctx->first_appear.end(), first_appear);
+  if (bfs) std::copy(ctx->bfs.begin(), ctx->bfs.end(), bfs);
+  if (level_off) std::copy(ctx->level_off.begin(), ctx->level_off.end(), level_off);
+  return H2H_OK;
+}
+
+h2h_status h2h_get_bags(const h2h_ctx* ctx, int64_t* off, int32_t* nbr, int32_t* w) {
+  if (!ctx) return H2H_ERR_ARG;
+  if (!ctx->tree_ready) return H2H_ERR_STATE;
+  if (off) std::copy(ctx->bag_off.begin(), ctx->bag_off.end(), off);
+  if (nbr) std::copy(ctx->bag_nbr.begin(), ctx->bag_nbr.end(), nbr);
+  if (w) std::copy(ctx->bag_w.begin(), ctx->bag_w.end(), w);
+  return H2H_OK;
+}
+
+h2h_status h2h_get_labels(h2h_ctx* ctx, int64_t* dis_hash, int32_t* dis, int64_t* pos_hash, int32_t* pos_id,
+                          int32_t* pos_pos) {
+  if (!ctx) return H2H_ERR_ARG;
+  if (!ctx->tree_ready) return H2H_ERR_STATE;
+  if (dis_hash) std::copy(ctx->dis_hash.begin(), ctx->dis_hash.end(), dis_hash);
+  if (pos_hash) std::copy(ctx->pos_hash.begin(), ctx->pos_hash.end(), pos_hash);
+  if (pos_id) std::copy(ctx->pos_id.begin(), ctx->pos_id.end(), pos_id);
+  if (pos_pos) std::copy(ctx->pos_pos.begin(), ctx->pos_pos.end(), pos_pos);
+  if (dis) {
+    if (!ctx->labels_ready) return set_err(ctx, H2H_ERR_STATE, "labels not built");
+    (void)hipSetDevice(ctx->device);
+    H2H_HIP(ctx, hipMemcpyAsync(dis, ctx->d_dis.p, ctx->label_entries * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    H2H_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return H2H_OK;
+}
+
+h2h_status h2h_write_construct_info(const h2h_ctx* ctx, const char* path, const char* graph_name,
+                                    int partition_method, int thread_num, int contract_device, int parallel_fine,
+                                    int construct_device) {
+  if (!ctx || !path || !graph_name) return H2H_ERR_ARG;
+  bool exists = false;
+  {
+    std::ifstream f(path);
+    exists = f.good();
+  }
+  std::ofstream out(path, std::ios::app);
+  if (!out.is_open()) return H2H_ERR_IO;
+  // column set of printConstructInfo (Graph_Fundamental.cu:148-161)
+  if (!exists)
+    out << "graphName,PartitionTreeHeight,changeHeight,ParititionTime,makePartitionRankTreeTime,AlllocateLUBTime,"
+           "LUBSize,translateLUBToGPUTime,TDInGPUTime,translateBackAndMallocToCPUTime,vertexesConstructedInGPU,"
+           "DecompositionTreeSize,TDInCPUTime,TDTREEHeight,,RMQsize,translateRMQTime,H2HLabelSize,,"
+           "changeConstructDeviceSize,H2HmallocTime,H2HConstructBFSTreeTime,H2HTranslateBFSTreeTime,"
+           "H2HUsingTime_CPU,H2HTranslateTime,H2HUsingTime_GPU,PartitionMethod,threadNum,contractDevice,"
+           "parallelFine,constructDevice\n";
+  const double MB = 1024.0 * 1024.0;
+  const int64_t posN = ctx->pos_hash.empty() ? 0 : ctx->pos_hash.back();
+  const double lub = ctx->lub_entries * 8.0 / MB;
+  const double dtree = posN * 12.0 / MB;
+  const double rmq = ctx->d_lca.bytes / MB;
+  const double label = ((ctx->n + 1) * 16.0 + posN * 4.0 + ctx->label_entries * 4.0) / MB;
+  out << graph_name << "," << ctx->H << "," << ctx->change_height << ",0," << (long long)ctx->t_partition_us << ",0,"
+      << lub << ",0," << (long long)ctx->t_contract_us << ",0," << ctx->top_base << "," << dtree << ",0,"
+      << ctx->tree_height << ",," << rmq << ",0," << label << ",,64," << (long long)ctx->t_tree_us << ",0,0,0,0,"
+      << (long long)ctx->t_label_us << "," << partition_method << "," << thread_num << "," << contract_device << ","
+      << parallel_fine << "," << construct_device << "\n";
+  return H2H_OK;
+}
+
+}  // extern "C"

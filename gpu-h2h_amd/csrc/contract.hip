@@ -1,0 +1,502 @@
+// Partition-level contraction on the GPU.
+//
+// One workgroup eliminates the interior vertices of one partition (all the
+// partitions of a level run in one launch; their vertex sets and every edge
+// their eliminations touch are disjoint).  The partition's current graph is
+// a dense n_p x n_p int32 weight block in HBM (row i = member i, ascending
+// vertex id), the per-member state (degree, height, flags) sits in LDS, and
+// each elimination is: block argmin of the packed key (deg+height, deg, id)
+// -> read the eliminated row -> drop it from its neighbours -> relax every
+// neighbour pair (fill-in).  The global top phase is the same kernel on one
+// block holding every remaining vertex.
+//
+// Elimination rule and keys: makeCHPerHeight_D_2 (kernel_functions.cu:310-479)
+// for H2H_CONTRACT_GPU_ORDER, partition_TD_T / CHUsingHeap_2
+// (Traditional_H2H.cu:216-351, 354-496) otherwise; the two differ in
+//  * the first pick of a partition, which the GPU kernel makes on the keys
+//    written when the partition rank tree was built (original degree,
+//    height 0; GraphPartition.cu:236) and propagates with height 0
+//    (kernel_functions.cu:325-329, 466-470), and
+//  * the slot-list degree: deleting the last entry of a list leaves its size
+//    unchanged (kernel_functions.cu:376-388), so that stale slot ("hole")
+//    counts in the degree key until the next deletion compacts the list.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <numeric>
+
+#include "h2h_internal.h"
+
+namespace h2h {
+namespace {
+
+constexpr int F_ALIVE = 1;
+constexpr int F_CAND = 2;
+constexpr int LDS_STATE_MAX = 2048;  // members whose state fits in LDS
+
+struct ContractArgs {
+  const int32_t* memb_off;   // [parts+1]
+  const int32_t* members;    // ascending id within a partition
+  const int32_t* outcnt;
+  const int32_t* slot;       // [n] index into members, -1 when not a member
+  const int64_t* w_off;      // [parts] offset of the dense block
+  const int32_t* elim_base;  // [parts]
+  const int32_t* ncand;      // [parts]
+  int32_t* W;
+  const int32_t* orig_deg;   // adjList sizes
+  int32_t* height;           // [n] CHInfo.second
+  const int64_t* cur_off;    // current adjacency rows
+  int32_t* cur_len;
+  const int32_t* cur_cap;
+  int32_t* cur_nbr;
+  int32_t* cur_w;
+  int32_t* elim;             // nonAdjcentNode ids
+  int32_t* rec_deg;
+  int32_t* rec_height;
+  int32_t* elim_pos;         // [n] slot of v in elim
+  int32_t* bag_len;          // [n]
+  int32_t* gstate;           // per-member state when it does not fit in LDS
+  int state_in_lds;
+  int stale_first;           // GPU-order rule: first pick on the stale keys
+  int track_holes;           // GPU-order rule: slot-list degree
+  int32_t* err;              // [1] bit 0: slot capacity, bit 1: key overflow
+};
+
+__device__ inline uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off, 64);
+    v = o < v ? o : v;
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
+  extern __shared__ int32_t smem[];
+  __shared__ uint64_t red[16];
+  __shared__ int32_t s_cnt;
+
+  const int part = blockIdx.x;
+  const int base = a.memb_off[part];
+  const int np = a.memb_off[part + 1] - base;
+  const int tid = threadIdx.x;
+  const int T = blockDim.x;
+  const int lane = tid & 63;
+  const int wid = tid >> 6;
+  const int nwaves = T >> 6;
+  int32_t* Wp = a.W + a.w_off[part];
+
+  int32_t* st = a.state_in_lds ? smem : (a.gstate + 6LL * base);
+  int32_t* s_deg = st;
+  int32_t* s_hgt = st + np;
+  int32_t* s_flag = st + 2 * np;
+  int32_t* s_holes = st + 3 * np;
+  int32_t* s_nb = st + 4 * np;
+  int32_t* s_nbw = st + 5 * np;
+
+  // ---- load: dense block from the current adjacency rows ----
+  const int64_t nn = (int64_t)np * np;
+  for (int64_t i = tid; i < nn; i += T) Wp[i] = INF32;
+  __syncthreads();
+  for (int i = tid; i < np; i += T) {
+    const int32_t g = a.members[base + i];
+    const int64_t off = a.cur_off[g];
+    const int32_t len = a.cur_len[g];
+    int32_t* row = Wp + (int64_t)i * np;
+    for (int32_t e = 0; e < len; e++) {
+      const int32_t s = a.slot[a.cur_nbr[off + e]];
+      if (s >= base && s < base + np) row[s - base] = a.cur_w[off + e];
+    }
+    s_deg[i] = len;
+    s_hgt[i] = a.height[g];
+    s_flag[i] = F_ALIVE | (a.outcnt[base + i] == 0 ? F_CAND : 0);
+    s_holes[i] = 0;
+  }
+  __syncthreads();
+
+  const int ncand = a.ncand[part];
+  const int ebase = a.elim_base[part];
+  for (int step = 0; step < ncand; step++) {
+    // ---- pick the minimum key among live candidates ----
+    const bool stale = a.stale_first && step == 0;
+    uint64_t best = ~0ull;
+    for (int i = tid; i < np; i += T) {
+      if (s_flag[i] == (F_ALIVE | F_CAND)) {
+        uint32_t dk, hk;
+        if (stale) {
+          dk = (uint32_t)a.orig_deg[a.members[base + i]];
+          hk = 0;
+        } else {
+          dk = (uint32_t)(s_deg[i] + s_holes[i]);
+          hk = (uint32_t)s_hgt[i];
+        }
+        const uint32_t dh = dk + hk;
+        uint64_t key = ((uint64_t)dh << 43) | ((uint64_t)(dk & 0x1fffff) << 22) | (uint64_t)i;
+        if (dh >= (1u << 21) || dk >= (1u << 21)) atomicOr(a.err, 2);
+        best = key < best ? key : best;
+      }
+    }
+    best = wave_min_u64(best);
+    if (lane == 0) red[wid] = best;
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+    best = red[0];
+    for (int w = 1; w < nwaves; w++) best = red[w] < best ? red[w] : best;
+    const int v = (int)(best & 0x3fffff);
+    const int dk = (int)((best >> 22) & 0x1fffff);
+    const int hk = (int)(best >> 43) - dk;
+    const int32_t gv = a.members[base + v];
+    // ---- gather the neighbours of v (its bag) ----
+    const int32_t* Wv = Wp + (int64_t)v * np;
+    for (int i = tid; i < np; i += T) {
+      if ((s_flag[i] & F_ALIVE) && i != v) {
+        const int32_t w = Wv[i];
+        if (w != INF32) {
+          const int k = atomicAdd(&s_cnt, 1);
+          s_nb[k] = i;
+          s_nbw[k] = w;
+        }
+      }
+    }
+    __syncthreads();
+    const int k = s_cnt;
+    if (tid == 0) {
+      const int pos = ebase + step;
+      a.elim[pos] = gv;
+      a.rec_deg[pos] = dk;
+      a.rec_height[pos] = hk;
+      a.elim_pos[gv] = pos;
+      a.bag_len[gv] = k;
+      s_flag[v] = F_CAND;  // no longer alive
+    }
+    // ---- remove v from its neighbours; height propagation ----
+    for (int t = tid; t < k; t += T) {
+      const int u = s_nb[t];
+      const int d = s_deg[u] - 1;
+      s_deg[u] = d;
+      if (a.track_holes) s_holes[u] = (d == 0) ? s_holes[u] + 1 : 0;
+      s_hgt[u] = max(s_hgt[u], hk + 1);
+    }
+    __syncthreads();
+    // ---- fill-in: relax every neighbour pair through v ----
+    for (int p = 0; p < k - 1; p++) {
+      const int u = s_nb[p];
+      const int wu = s_nbw[p];
+      int32_t* rowu = Wp + (int64_t)u * np;
+      for (int q = p + 1 + tid; q < k; q += T) {
+        const int x = s_nb[q];
+        const int ws = wu + s_nbw[q];
+        int32_t* cux = rowu + x;
+        int32_t* cxu = Wp + (int64_t)x * np + u;
+        const int32_t cur = *cux;
+        if (cur == INF32) {
+          *cux = ws;
+          *cxu = ws;
+          atomicAdd(&s_deg[u], 1);
+          atomicAdd(&s_deg[x], 1);
+        } else {
+          if (ws < cur) *cux = ws;
+          if (ws < *cxu) *cxu = ws;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- write back the surviving (boundary) members' adjacency rows ----
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int i = wid; i < np; i += nwaves) {
+    const int32_t g = a.members[base + i];
+    if (lane == 0) a.height[g] = s_hgt[i];
+    if (s_flag[i] != F_ALIVE) continue;  // candidates are eliminated
+    const int64_t off = a.cur_off[g];
+    const int32_t len = a.cur_len[g];
+    const int32_t cap = a.cur_cap[g];
+    int outp = 0;
+    for (int c = 0; c < len; c += 64) {
+      const int e = c + lane;
+      bool keep = false;
+      int32_t u = 0, w = 0;
+      if (e < len) {
+        u = a.cur_nbr[off + e];
+        w = a.cur_w[off + e];
+        const int32_t s = a.slot[u];
+        keep = !(s >= base && s < base + np);
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int pos = outp + __popcll(m & lt);
+        a.cur_nbr[off + pos] = u;
+        a.cur_w[off + pos] = w;
+      }
+      outp += __popcll(m);
+    }
+    const int32_t* Wi = Wp + (int64_t)i * np;
+    for (int c = 0; c < np; c += 64) {
+      const int j = c + lane;
+      bool keep = false;
+      int32_t w = 0;
+      if (j < np && j != i && (s_flag[j] & F_ALIVE)) {
+        w = Wi[j];
+        keep = (w != INF32);
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int pos = outp + __popcll(m & lt);
+        if (pos < cap) {
+          a.cur_nbr[off + pos] = a.members[base + j];
+          a.cur_w[off + pos] = w;
+        } else {
+          atomicOr(a.err, 1);
+        }
+      }
+      outp += __popcll(m);
+    }
+    if (lane == 0) a.cur_len[g] = min(outp, cap);
+  }
+}
+
+// Bags of the eliminated members: the row of v is frozen once v is gone, so
+// its bag is every entry whose member was still present when v was picked.
+struct BagArgs {
+  const int32_t* memb_off;
+  const int32_t* members;
+  const int32_t* outcnt;
+  const int64_t* w_off;
+  const int32_t* W;
+  const int32_t* elim_pos;
+  const int64_t* bag_off;
+  int32_t* bag_nbr;
+  int32_t* bag_w;
+};
+
+__global__ void __launch_bounds__(256) extract_bags_kernel(BagArgs a) {
+  const int part = blockIdx.x;
+  const int base = a.memb_off[part];
+  const int np = a.memb_off[part + 1] - base;
+  const int lane = threadIdx.x & 63;
+  const int wid = threadIdx.x >> 6;
+  const int nwaves = blockDim.x >> 6;
+  const int32_t* Wp = a.W + a.w_off[part];
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int i = wid; i < np; i += nwaves) {
+    if (a.outcnt[base + i] != 0) continue;
+    const int32_t gv = a.members[base + i];
+    const int32_t pv = a.elim_pos[gv];
+    const int64_t out = a.bag_off[gv];
+    const int32_t* Wi = Wp + (int64_t)i * np;
+    int outp = 0;
+    for (int c = 0; c < np; c += 64) {
+      const int j = c + lane;
+      bool keep = false;
+      int32_t w = 0, gj = 0;
+      if (j < np && j != i) {
+        w = Wi[j];
+        if (w != INF32) {
+          gj = a.members[base + j];
+          keep = (a.outcnt[base + j] != 0) || (a.elim_pos[gj] > pv);
+        }
+      }
+      const uint64_t m = __ballot(keep);
+      if (keep) {
+        const int pos = outp + __popcll(m & lt);
+        a.bag_nbr[out + pos] = gj;
+        a.bag_w[out + pos] = w;
+      }
+      outp += __popcll(m);
+    }
+  }
+}
+
+template <class T>
+h2h_status up(h2h_ctx* ctx, DevArray& d, const std::vector<T>& h) {
+  return upload(ctx, d, h.data(), h.size() * sizeof(T));
+}
+
+struct PlanDev {
+  DevArray memb_off, members, outcnt, slot, w_off, elim_base, ncand, gstate;
+  bool lds = true;
+  int block = 256;
+  size_t shmem = 0;
+};
+
+}  // namespace
+
+#define H2H_TRY(x)                     \
+  do {                                 \
+    h2h_status s_ = (x);               \
+    if (s_ != H2H_OK) return s_;       \
+  } while (0)
+
+h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
+  const int n = ctx->n;
+  hipStream_t s = ctx->stream;
+  // ---- current adjacency rows (capacity from the partition plan) ----
+  const std::vector<int32_t>& cap = ctx->cur_cap;
+  std::vector<int64_t> cur_off(n);
+  std::vector<int32_t> cur_len(n), orig_deg(n);
+  int64_t tot = 0;
+  for (int v = 0; v < n; v++) {
+    cur_off[v] = tot;
+    orig_deg[v] = (int32_t)(ctx->adj_off[v + 1] - ctx->adj_off[v]);
+    cur_len[v] = orig_deg[v];
+    tot += std::max(cap[v], orig_deg[v]);
+  }
+  std::vector<int32_t> cur_nbr(std::max<int64_t>(tot, 1)), cur_w(std::max<int64_t>(tot, 1));
+  for (int v = 0; v < n; v++) {
+    std::copy(ctx->adj_nbr.begin() + ctx->adj_off[v], ctx->adj_nbr.begin() + ctx->adj_off[v + 1], cur_nbr.begin() + cur_off[v]);
+    std::copy(ctx->adj_w.begin() + ctx->adj_off[v], ctx->adj_w.begin() + ctx->adj_off[v + 1], cur_w.begin() + cur_off[v]);
+  }
+  DevArray d_cur_off, d_cur_len, d_cur_cap, d_cur_nbr, d_cur_w, d_orig, d_height;
+  DevArray d_elim, d_rdeg, d_rhgt, d_epos, d_blen, d_err, d_W;
+  H2H_TRY(up(ctx, d_cur_off, cur_off));
+  H2H_TRY(up(ctx, d_cur_len, cur_len));
+  H2H_TRY(up(ctx, d_cur_cap, cap));
+  H2H_TRY(up(ctx, d_cur_nbr, cur_nbr));
+  H2H_TRY(up(ctx, d_cur_w, cur_w));
+  H2H_TRY(up(ctx, d_orig, orig_deg));
+  H2H_TRY(dev_alloc(ctx, d_height, n * sizeof(int32_t)));
+  H2H_HIP(ctx, hipMemsetAsync(d_height.p, 0, n * sizeof(int32_t), s));
+  // elim starts as the partition-rank-tree record (top-phase slots are rewritten)
+  H2H_TRY(up(ctx, d_elim, ctx->elim));
+  H2H_TRY(dev_alloc(ctx, d_rdeg, n * sizeof(int32_t)));
+  H2H_TRY(dev_alloc(ctx, d_rhgt, n * sizeof(int32_t)));
+  H2H_TRY(dev_alloc(ctx, d_epos, n * sizeof(int32_t)));
+  H2H_TRY(dev_alloc(ctx, d_blen, n * sizeof(int32_t)));
+  H2H_TRY(dev_alloc(ctx, d_err, sizeof(int32_t)));
+  H2H_HIP(ctx, hipMemsetAsync(d_blen.p, 0, n * sizeof(int32_t), s));
+  H2H_HIP(ctx, hipMemsetAsync(d_err.p, 0, sizeof(int32_t), s));
+
+  // ---- per-plan inputs ----
+  int64_t wtot = 0;
+  std::vector<int64_t> wbase(ctx->plans.size());
+  for (size_t pi = 0; pi < ctx->plans.size(); pi++) {
+    wbase[pi] = wtot;
+    wtot += ctx->plans[pi].w_total;
+  }
+  if (wtot > (int64_t)16 << 30)
+    return set_err(ctx, H2H_ERR_OOM, "dense contraction blocks exceed 64 GiB (top phase too large; lower ChangeToGlobalHeight)");
+  H2H_TRY(dev_alloc(ctx, d_W, std::max<int64_t>(wtot, 1) * sizeof(int32_t)));
+  std::vector<PlanDev> pd(ctx->plans.size());
+  for (size_t pi = 0; pi < ctx->plans.size(); pi++) {
+    const LevelPlan& P = ctx->plans[pi];
+    PlanDev& D = pd[pi];
+    std::vector<int32_t> slot(n, -1);
+    for (size_t i = 0; i < P.members.size(); i++) slot[P.members[i]] = (int32_t)i;
+    std::vector<int64_t> woff(P.w_off.size());
+    for (size_t k = 0; k < woff.size(); k++) woff[k] = P.w_off[k] + wbase[pi];
+    H2H_TRY(up(ctx, D.memb_off, P.memb_off));
+    H2H_TRY(up(ctx, D.members, P.members));
+    H2H_TRY(up(ctx, D.outcnt, P.outcnt));
+    H2H_TRY(up(ctx, D.slot, slot));
+    H2H_TRY(up(ctx, D.w_off, woff));
+    H2H_TRY(up(ctx, D.elim_base, P.elim_base));
+    H2H_TRY(up(ctx, D.ncand, P.ncand));
+    D.lds = P.max_np <= LDS_STATE_MAX;
+    D.block = (P.height == 0) ? 1024 : 256;
+    D.shmem = D.lds ? (size_t)6 * P.max_np * sizeof(int32_t) : 0;
+    if (!D.lds) H2H_TRY(dev_alloc(ctx, D.gstate, (size_t)6 * P.members.size() * sizeof(int32_t)));
+  }
+
+  hipEvent_t e0, e1;
+  H2H_HIP(ctx, hipEventCreate(&e0));
+  H2H_HIP(ctx, hipEventCreate(&e1));
+  H2H_HIP(ctx, hipEventRecord(e0, s));
+  const bool gpu_order = ctx->contract_order == H2H_CONTRACT_GPU_ORDER;
+  for (size_t pi = 0; pi < ctx->plans.size(); pi++) {
+    const LevelPlan& P = ctx->plans[pi];
+    PlanDev& D = pd[pi];
+    ContractArgs a;
+    a.memb_off = D.memb_off.as<int32_t>();
+    a.members = D.members.as<int32_t>();
+    a.outcnt = D.outcnt.as<int32_t>();
+    a.slot = D.slot.as<int32_t>();
+    a.w_off = D.w_off.as<int64_t>();
+    a.elim_base = D.elim_base.as<int32_t>();
+    a.ncand = D.ncand.as<int32_t>();
+    a.W = d_W.as<int32_t>();
+    a.orig_deg = d_orig.as<int32_t>();
+    a.height = d_height.as<int32_t>();
+    a.cur_off = d_cur_off.as<int64_t>();
+    a.cur_len = d_cur_len.as<int32_t>();
+    a.cur_cap = d_cur_cap.as<int32_t>();
+    a.cur_nbr = d_cur_nbr.as<int32_t>();
+    a.cur_w = d_cur_w.as<int32_t>();
+    a.elim = d_elim.as<int32_t>();
+    a.rec_deg = d_rdeg.as<int32_t>();
+    a.rec_height = d_rhgt.as<int32_t>();
+    a.elim_pos = d_epos.as<int32_t>();
+    a.bag_len = d_blen.as<int32_t>();
+    a.gstate = D.gstate.as<int32_t>();
+    a.state_in_lds = D.lds ? 1 : 0;
+    a.stale_first = (gpu_order && P.height > 0) ? 1 : 0;
+    a.track_holes = (gpu_order && P.height > 0) ? 1 : 0;
+    a.err = d_err.as<int32_t>();
+    hipLaunchKernelGGL(contract_kernel, dim3((unsigned)P.heap.size()), dim3(D.block), D.shmem, s, a);
+    H2H_HIP(ctx, hipGetLastError());
+  }
+  // ---- bags: sizes -> offsets -> extraction ----
+  std::vector<int32_t> blen(n);
+  H2H_HIP(ctx, hipMemcpyAsync(blen.data(), d_blen.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  int32_t herr = 0;
+  H2H_HIP(ctx, hipMemcpyAsync(&herr, d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipStreamSynchronize(s));
+  if (herr & 1) return set_err(ctx, H2H_ERR_LUB, "a boundary vertex outgrew its slot capacity");
+  if (herr & 2) return set_err(ctx, H2H_ERR_INPUT, "elimination key exceeds 21 bits (degree + height)");
+  ctx->bag_off.assign(n + 1, 0);
+  for (int v = 0; v < n; v++) ctx->bag_off[v + 1] = ctx->bag_off[v] + blen[v];
+  const int64_t nb = ctx->bag_off[n];
+  DevArray d_boff, d_bnbr, d_bw;
+  H2H_TRY(up(ctx, d_boff, ctx->bag_off));
+  H2H_TRY(dev_alloc(ctx, d_bnbr, std::max<int64_t>(nb, 1) * sizeof(int32_t)));
+  H2H_TRY(dev_alloc(ctx, d_bw, std::max<int64_t>(nb, 1) * sizeof(int32_t)));
+  for (size_t pi = 0; pi < ctx->plans.size(); pi++) {
+    BagArgs b;
+    b.memb_off = pd[pi].memb_off.as<int32_t>();
+    b.members = pd[pi].members.as<int32_t>();
+    b.outcnt = pd[pi].outcnt.as<int32_t>();
+    b.w_off = pd[pi].w_off.as<int64_t>();
+    b.W = d_W.as<int32_t>();
+    b.elim_pos = d_epos.as<int32_t>();
+    b.bag_off = d_boff.as<int64_t>();
+    b.bag_nbr = d_bnbr.as<int32_t>();
+    b.bag_w = d_bw.as<int32_t>();
+    hipLaunchKernelGGL(extract_bags_kernel, dim3((unsigned)ctx->plans[pi].heap.size()), dim3(256), 0, s, b);
+    H2H_HIP(ctx, hipGetLastError());
+  }
+  H2H_HIP(ctx, hipEventRecord(e1, s));
+  ctx->bag_nbr.resize(nb);
+  ctx->bag_w.resize(nb);
+  ctx->rec_deg.resize(n);
+  ctx->rec_height.resize(n);
+  ctx->elim.resize(n);
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->bag_nbr.data(), d_bnbr.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->bag_w.data(), d_bw.p, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->elim.data(), d_elim.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->rec_deg.data(), d_rdeg.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->rec_height.data(), d_rhgt.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  H2H_HIP(ctx, hipStreamSynchronize(s));
+  float ms = 0;
+  H2H_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx->t_contract_us = ms * 1000.0;
+  if (usec) *usec = ctx->t_contract_us;
+  // sort each bag by neighbour id (the device writes them in column order,
+  // which is ascending id inside a partition already)
+  for (int v = 0; v < n; v++) {
+    int64_t b0 = ctx->bag_off[v], b1 = ctx->bag_off[v + 1];
+    bool sorted = true;
+    for (int64_t e = b0 + 1; e < b1; e++)
+      if (ctx->bag_nbr[e - 1] > ctx->bag_nbr[e]) { sorted = false; break; }
+    if (!sorted) {
+      std::vector<std::pair<int32_t, int32_t>> t;
+      for (int64_t e = b0; e < b1; e++) t.emplace_back(ctx->bag_nbr[e], ctx->bag_w[e]);
+      std::sort(t.begin(), t.end());
+      for (int64_t e = b0; e < b1; e++) { ctx->bag_nbr[e] = t[e - b0].first; ctx->bag_w[e] = t[e - b0].second; }
+    }
+  }
+  return H2H_OK;
+}
+
+}  // namespace h2h

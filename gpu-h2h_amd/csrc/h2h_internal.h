@@ -1,0 +1,162 @@
+// Internal state of an h2h_ctx and the helpers shared by the host and HIP
+// translation units of libh2h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "../../include/h2h.h"
+
+namespace h2h {
+
+constexpr int32_t INF32 = 0x7fffffff;
+
+// Owning device allocation.
+struct DevArray {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  DevArray(DevArray&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevArray& operator=(DevArray&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// One partition level of the contraction as the device kernel sees it.
+struct LevelPlan {
+  int height = 0;                  // partition level h (0 = global top phase)
+  std::vector<int32_t> heap;       // heap index of each partition (-1 for the top phase)
+  std::vector<int32_t> memb_off;   // [parts+1] offsets into members
+  std::vector<int32_t> members;    // present vertices, ascending id inside a partition
+  std::vector<int32_t> outcnt;     // present neighbours outside the partition (0 = candidate)
+  std::vector<int32_t> elim_base;  // [parts] first nonAdjcentNode slot of the partition
+  std::vector<int32_t> ncand;      // [parts] interior vertices
+  std::vector<int64_t> w_off;      // [parts] offset of the dense weight block (int32 units)
+  int64_t w_total = 0;
+  int max_np = 0;
+};
+
+// Device scratch of one batched query launch (one per stream that runs them).
+struct QueryWorkspace {
+  DevArray lca, cls, order, ctr;
+  int64_t cap = 0;
+};
+
+struct Request;   // serving
+struct Server;
+
+}  // namespace h2h
+
+struct h2h_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // ---- graph (Graph::Graph) ----
+  int32_t n = 0;
+  int64_t arcs = 0;
+  std::vector<int64_t> adj_off;  // [n+1]
+  std::vector<int32_t> adj_nbr, adj_w;
+  std::vector<int32_t> index_ids, id_hash;
+  int H = 0;                     // partition height after min(file, arg)
+  std::vector<int32_t> cut_lr;   // 2 * (2^H - 1)
+  bool graph_loaded = false;
+
+  // ---- partition rank tree + contraction ----
+  int change_height = 0;
+  int contract_order = 1;
+  std::vector<int32_t> nan_start, nan_max, nan_count;  // per heap node
+  std::vector<int32_t> level_of;                       // partition level where v is interior
+  std::vector<int32_t> cur_cap;                        // adjacency row capacity during contraction
+  std::vector<double> layer_fraction;
+  std::vector<h2h::LevelPlan> plans;                   // partition levels H..C+1, then the top phase
+  std::vector<int32_t> elim, rec_deg, rec_height;      // nonAdjcentNode
+  int64_t lub_entries = 0;
+  int64_t top_vertices = 0;
+  int32_t top_base = 0;
+  bool contracted = false;
+
+  // ---- tree and label layout (host copies) ----
+  std::vector<int64_t> bag_off;   // [n+1]
+  std::vector<int32_t> bag_nbr, bag_w;
+  std::vector<int32_t> rank, father, depth, first_appear, bfs, level_off;
+  std::vector<int32_t> preorder, tin;
+  std::vector<int64_t> dis_hash, pos_hash;
+  std::vector<int32_t> pos_id, pos_pos, pos_w;
+  int32_t head = -1;
+  int32_t tree_height = 0;
+  int64_t label_entries = 0;
+  bool tree_ready = false;
+  bool labels_ready = false;
+  int label_kind = 4;
+
+  // ---- resident device index ----
+  h2h::DevArray d_dis_hash, d_dis, d_pos_hash, d_pos_pos, d_pos_w, d_pos_rowoff;
+  h2h::DevArray d_depth, d_bfs, d_father, d_anc, d_lca, d_tin;
+  h2h::DevArray d_qx, d_qy, d_qout;   // staging for host-buffer queries
+  size_t q_cap = 0;
+  h2h::QueryWorkspace qws;
+  int lca_levels = 0;
+  int num_cus = 256;
+
+  // ---- timings ----
+  double t_partition_us = 0, t_contract_us = 0, t_tree_us = 0, t_label_us = 0;
+
+  // ---- serving ----
+  std::unique_ptr<h2h::Server> server;
+
+  h2h_ctx();
+  ~h2h_ctx();
+};
+
+namespace h2h {
+
+// error helpers: record a message on the context and return the status
+h2h_status set_err(h2h_ctx* ctx, h2h_status st, const std::string& msg);
+h2h_status hip_err(h2h_ctx* ctx, hipError_t e, const char* what);
+
+#define H2H_HIP(ctx, call)                                    \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return ::h2h::hip_err(ctx, e_, #call); \
+  } while (0)
+
+h2h_status dev_alloc(h2h_ctx* ctx, DevArray& a, size_t bytes);
+h2h_status upload(h2h_ctx* ctx, DevArray& a, const void* src, size_t bytes);
+
+// host stages (host_graph.cpp)
+h2h_status build_partition_plans(h2h_ctx* ctx, int change_height);
+h2h_status build_tree_and_layout(h2h_ctx* ctx);
+void merge_cpu_order(h2h_ctx* ctx, int thread_num);
+
+// device stages
+h2h_status run_contraction(h2h_ctx* ctx, double* usec);               // contract.hip
+h2h_status upload_index(h2h_ctx* ctx);                                 // labels.hip
+h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec);  // labels.hip
+h2h_status build_lca(h2h_ctx* ctx);                                    // query.hip
+h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, const int32_t* dy,
+                        int32_t* dout, int64_t n, hipStream_t s, QueryWorkspace* ws);  // query.hip
+h2h_status query_workspace(h2h_ctx* ctx, QueryWorkspace& ws, int64_t n);             // query.hip
+
+}  // namespace h2h

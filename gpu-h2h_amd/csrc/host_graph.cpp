@@ -1,0 +1,330 @@
+// Host stages of the H2H pipeline: input checking, the partition rank tree
+// (which vertices each partition level eliminates), and the tree / label
+// layout built from the bags the device contraction produced.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstring>
+#include <numeric>
+#include <queue>
+
+#include "h2h_internal.h"
+
+namespace h2h {
+
+h2h_status set_err(h2h_ctx* ctx, h2h_status st, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return st;
+}
+
+h2h_status hip_err(h2h_ctx* ctx, hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return set_err(ctx, e == hipErrorOutOfMemory ? H2H_ERR_OOM : H2H_ERR_HIP, m);
+}
+
+h2h_status dev_alloc(h2h_ctx* ctx, DevArray& a, size_t bytes) {
+  if (a.p && a.bytes >= bytes) return H2H_OK;
+  a.release();
+  if (bytes == 0) bytes = 4;
+  hipError_t e = hipMalloc(&a.p, bytes);
+  if (e != hipSuccess) {
+    a.p = nullptr;
+    return hip_err(ctx, e, "hipMalloc");
+  }
+  a.bytes = bytes;
+  return H2H_OK;
+}
+
+h2h_status upload(h2h_ctx* ctx, DevArray& a, const void* src, size_t bytes) {
+  h2h_status st = dev_alloc(ctx, a, bytes);
+  if (st != H2H_OK) return st;
+  if (bytes) H2H_HIP(ctx, hipMemcpyAsync(a.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  return H2H_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Partition rank tree.  Same result as makeAdjcentNode (GraphPartition.cu:6-136):
+// levels H..1, partitions of a level from the highest heap index down; a
+// vertex still present in [l, r) is interior when none of its arcs reaches a
+// present vertex outside the range; interior vertices are appended in
+// ascending id order.  Stamps replace the per-partition std::map.
+// Also plans the device work: the present members of every partition at the
+// levels H..C+1 with their outside-neighbour counts, and the global top phase.
+// ---------------------------------------------------------------------------
+h2h_status build_partition_plans(h2h_ctx* ctx, int C) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int n = ctx->n;
+  const int H = ctx->H;
+  const int64_t heapN = (1LL << H) - 1;
+  ctx->nan_start.assign(heapN, 0);
+  ctx->nan_max.assign(heapN, 0);
+  ctx->nan_count.assign(heapN, 0);
+  ctx->level_of.assign(n, 0);
+  ctx->elim.clear();
+  ctx->elim.reserve(n);
+  ctx->plans.clear();
+  ctx->layer_fraction.assign(H, 0.0);
+
+  std::vector<char> present(n, 1);
+  std::vector<int32_t> stamp(n, -1);
+  std::vector<int32_t> cap(n, 0);
+  for (int v = 0; v < n; v++) cap[v] = (int32_t)(ctx->adj_off[v + 1] - ctx->adj_off[v]);
+  std::vector<int32_t> members, outc;
+  ctx->top_base = -1;
+
+  for (int h = H; h >= 1; h--) {
+    const int64_t lo = (1LL << (h - 1)) - 1, hi = (1LL << h) - 2;
+    std::vector<int32_t> newly;
+    LevelPlan plan;
+    plan.height = h;
+    plan.memb_off.push_back(0);
+    if (h == C) ctx->top_base = (int32_t)ctx->elim.size();
+    for (int64_t p = hi; p >= lo; p--) {
+      int32_t l = ctx->cut_lr[2 * p], r = ctx->cut_lr[2 * p + 1];
+      members.clear();
+      for (int32_t j = l; j < r; j++) {
+        int32_t v = ctx->index_ids[j];
+        if (present[v]) members.push_back(v);
+      }
+      std::sort(members.begin(), members.end());
+      for (int32_t v : members) stamp[v] = (int32_t)p;
+      outc.assign(members.size(), 0);
+      int32_t ncand = 0;
+      for (size_t i = 0; i < members.size(); i++) {
+        int32_t v = members[i];
+        int32_t c = 0;
+        for (int64_t e = ctx->adj_off[v]; e < ctx->adj_off[v + 1]; e++) {
+          int32_t u = ctx->adj_nbr[e];
+          if (present[u] && stamp[u] != (int32_t)p) c++;
+        }
+        outc[i] = c;
+        if (c == 0) ncand++;
+      }
+      ctx->nan_start[p] = (int32_t)ctx->elim.size();
+      ctx->nan_max[p] = (int32_t)members.size();
+      ctx->nan_count[p] = ncand;
+      for (size_t i = 0; i < members.size(); i++) {
+        int32_t v = members[i];
+        if (outc[i] == 0) {
+          ctx->elim.push_back(v);
+          ctx->level_of[v] = h;
+          newly.push_back(v);
+        }
+      }
+      if (h > C && ncand > 0) {
+        const int32_t np = (int32_t)members.size();
+        const int32_t survivors = np - ncand;
+        for (size_t i = 0; i < members.size(); i++)
+          if (outc[i] > 0) cap[members[i]] = std::max(cap[members[i]], survivors - 1 + outc[i]);
+        plan.heap.push_back((int32_t)p);
+        plan.members.insert(plan.members.end(), members.begin(), members.end());
+        plan.outcnt.insert(plan.outcnt.end(), outc.begin(), outc.end());
+        plan.memb_off.push_back((int32_t)plan.members.size());
+        plan.elim_base.push_back(ctx->nan_start[p]);
+        plan.ncand.push_back(ncand);
+        plan.w_off.push_back(plan.w_total);
+        plan.w_total += (int64_t)np * np;
+        plan.max_np = std::max(plan.max_np, np);
+      }
+    }
+    for (int32_t v : newly) present[v] = 0;
+    ctx->layer_fraction[h - 1] = (double)ctx->elim.size() / n;
+    if (h > C && !plan.heap.empty()) ctx->plans.push_back(std::move(plan));
+  }
+  if ((int64_t)ctx->elim.size() != n)
+    return set_err(ctx, H2H_ERR_INPUT, "the cut heap leaves vertices outside every partition");
+
+  // global top phase: every vertex interior at a level <= C, one block
+  LevelPlan top;
+  top.height = 0;
+  top.heap.push_back(-1);
+  top.memb_off.push_back(0);
+  for (int32_t i = ctx->top_base; i < n; i++) top.members.push_back(ctx->elim[i]);
+  std::sort(top.members.begin(), top.members.end());
+  top.outcnt.assign(top.members.size(), 0);
+  top.memb_off.push_back((int32_t)top.members.size());
+  top.elim_base.push_back(ctx->top_base);
+  top.ncand.push_back((int32_t)top.members.size());
+  top.w_off.push_back(0);
+  top.w_total = (int64_t)top.members.size() * (int64_t)top.members.size();
+  top.max_np = (int)top.members.size();
+  ctx->top_vertices = (int64_t)top.members.size();
+  ctx->plans.push_back(std::move(top));
+
+  ctx->lub_entries = 0;
+  for (int v = 0; v < n; v++) ctx->lub_entries += cap[v];
+  ctx->cur_cap.swap(cap);
+  ctx->t_partition_us =
+      std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  return H2H_OK;
+}
+
+// ---------------------------------------------------------------------------
+// partition_TD_T's record order (Traditional_H2H.cu:381-476): partitions of a
+// level are dealt to thread_num threads in chunks of consecutive heap indices;
+// a thread repeatedly eliminates the minimum key over its whole chunk and
+// writes it to the next slot of the chunk's region.  Partitions are
+// independent, so that record is the merge of the per-partition sequences by
+// the key each vertex had when picked.  The tree does not depend on it.
+// ---------------------------------------------------------------------------
+void merge_cpu_order(h2h_ctx* ctx, int T) {
+  if (T < 1) T = 1;
+  const int H = ctx->H, C = ctx->change_height;
+  for (int h = H; h > C; h--) {
+    const int64_t lo = (1LL << (h - 1)) - 1, hiEx = (1LL << h) - 1;
+    const int64_t total = hiEx - lo;
+    const int64_t per = total / T + 1;
+    for (int64_t a = lo; a < hiEx; a += per) {
+      int64_t b = std::min(hiEx, a + per);
+      int32_t start = INT_MAX, end = 0;
+      for (int64_t p = a; p < b; p++) {
+        start = std::min(start, ctx->nan_start[p]);
+        end = std::max(end, ctx->nan_start[p] + ctx->nan_count[p]);
+      }
+      if (start >= end) continue;
+      using Key = std::tuple<int64_t, int32_t, int32_t, int64_t>;  // (deg+h, deg, id, partition)
+      std::priority_queue<Key, std::vector<Key>, std::greater<Key>> pq;
+      std::vector<int32_t> cursor(b - a, 0);
+      auto push = [&](int64_t p) {
+        int32_t c = cursor[p - a];
+        if (c >= ctx->nan_count[p]) return;
+        int32_t s = ctx->nan_start[p] + c;
+        pq.emplace((int64_t)ctx->rec_deg[s] + ctx->rec_height[s], ctx->rec_deg[s], ctx->elim[s], p);
+      };
+      for (int64_t p = a; p < b; p++) push(p);
+      std::vector<int32_t> e2, d2, h2;
+      while (!pq.empty()) {
+        Key k = pq.top();
+        pq.pop();
+        int64_t p = std::get<3>(k);
+        int32_t s = ctx->nan_start[p] + cursor[p - a];
+        e2.push_back(ctx->elim[s]);
+        d2.push_back(ctx->rec_deg[s]);
+        h2.push_back(ctx->rec_height[s]);
+        cursor[p - a]++;
+        push(p);
+      }
+      std::copy(e2.begin(), e2.end(), ctx->elim.begin() + start);
+      std::copy(d2.begin(), d2.end(), ctx->rec_deg.begin() + start);
+      std::copy(h2.begin(), h2.end(), ctx->rec_height.begin() + start);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tree and label layout from the bags: getrank / getTreeStructure /
+// OULADFS (TreeStructure.cu:102-163, 242-262), mallocH2HLabel_noHub
+// (H2HConsturction.cu:618-722, pos sorted by position, self last) and
+// constructBFSTree (H2HConsturction.cu:324-377).
+// ---------------------------------------------------------------------------
+h2h_status build_tree_and_layout(h2h_ctx* ctx) {
+  auto t0 = std::chrono::steady_clock::now();
+  const int n = ctx->n;
+  ctx->rank.assign(n, -1);
+  for (int i = 0; i < n; i++) ctx->rank[ctx->elim[i]] = i;
+  ctx->head = ctx->elim[n - 1];
+  ctx->father.assign(n, -1);
+  for (int v = 0; v < n; v++) {
+    int32_t f = -1, r = INT_MAX;
+    for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
+      int32_t u = ctx->bag_nbr[e];
+      if (ctx->rank[u] < r) { r = ctx->rank[u]; f = u; }
+    }
+    ctx->father[v] = f;
+  }
+  // children CSR in ascending child id
+  std::vector<int32_t> ccount(n + 1, 0), child(std::max(n - 1, 1));
+  for (int v = 0; v < n; v++)
+    if (ctx->father[v] >= 0) ccount[ctx->father[v] + 1]++;
+  for (int v = 0; v < n; v++) ccount[v + 1] += ccount[v];
+  {
+    std::vector<int32_t> fill(ccount.begin(), ccount.end() - 1);
+    for (int v = 0; v < n; v++)
+      if (ctx->father[v] >= 0) child[fill[ctx->father[v]]++] = v;
+  }
+  // DFS: depth, preorder, Euler first appearance
+  ctx->depth.assign(n, 0);
+  ctx->first_appear.assign(n, -1);
+  ctx->preorder.clear();
+  ctx->preorder.reserve(n);
+  ctx->tin.assign(n, -1);
+  int64_t euler = 0;
+  int32_t maxd = 0;
+  int64_t sumd = 0;
+  std::vector<std::pair<int32_t, int32_t>> st;  // (node, next child cursor)
+  auto enter = [&](int32_t v, int32_t d) {
+    ctx->depth[v] = d;
+    ctx->first_appear[v] = (int32_t)euler++;
+    ctx->tin[v] = (int32_t)ctx->preorder.size();
+    ctx->preorder.push_back(v);
+    maxd = std::max(maxd, d);
+    sumd += d;
+    st.emplace_back(v, ccount[v]);
+  };
+  enter(ctx->head, 1);
+  while (!st.empty()) {
+    auto& top = st.back();
+    int32_t v = top.first;
+    if (top.second < ccount[v + 1]) {
+      int32_t c = child[top.second++];
+      enter(c, ctx->depth[v] + 1);
+    } else {
+      st.pop_back();
+      if (!st.empty()) euler++;
+    }
+  }
+  if ((int)ctx->preorder.size() != n)
+    return set_err(ctx, H2H_ERR_INPUT, "elimination tree is a forest (graph disconnected)");
+  ctx->tree_height = maxd;
+  ctx->label_entries = sumd;
+
+  // label layout
+  ctx->dis_hash.assign(n + 1, 0);
+  for (int v = 0; v < n; v++) ctx->dis_hash[v + 1] = ctx->dis_hash[v] + ctx->depth[v];
+  ctx->pos_hash.assign(n + 1, 0);
+  for (int v = 0; v < n; v++) ctx->pos_hash[v + 1] = ctx->pos_hash[v] + (ctx->bag_off[v + 1] - ctx->bag_off[v]) + 1;
+  const int64_t P = ctx->pos_hash[n];
+  ctx->pos_id.assign(P, 0);
+  ctx->pos_pos.assign(P, 0);
+  ctx->pos_w.assign(P, 0);
+  std::vector<std::tuple<int32_t, int32_t, int32_t>> ent;
+  for (int v = 0; v < n; v++) {
+    ent.clear();
+    for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
+      int32_t u = ctx->bag_nbr[e];
+      ent.emplace_back(ctx->depth[u] - 1, u, ctx->bag_w[e]);
+    }
+    std::sort(ent.begin(), ent.end());
+    int64_t k = ctx->pos_hash[v];
+    for (auto& t : ent) {
+      ctx->pos_pos[k] = std::get<0>(t);
+      ctx->pos_id[k] = std::get<1>(t);
+      ctx->pos_w[k] = std::get<2>(t);
+      k++;
+    }
+    ctx->pos_pos[k] = ctx->depth[v] - 1;
+    ctx->pos_id[k] = v;
+    ctx->pos_w[k] = 0;
+  }
+  // BFS levels (TreeBFS / TreeHash)
+  ctx->bfs.clear();
+  ctx->bfs.reserve(n);
+  ctx->level_off.assign(1, 0);
+  ctx->bfs.push_back(ctx->head);
+  ctx->level_off.push_back(1);
+  size_t a = 0;
+  while (a < ctx->bfs.size()) {
+    size_t b = ctx->bfs.size();
+    for (size_t i = a; i < b; i++) {
+      int32_t v = ctx->bfs[i];
+      for (int32_t c = ccount[v]; c < ccount[v + 1]; c++) ctx->bfs.push_back(child[c]);
+    }
+    a = b;
+    if (ctx->bfs.size() > b) ctx->level_off.push_back((int32_t)ctx->bfs.size());
+  }
+  ctx->tree_ready = true;
+  ctx->t_tree_us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  return H2H_OK;
+}
+
+}  // namespace h2h

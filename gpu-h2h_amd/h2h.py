@@ -1,0 +1,297 @@
+"""Python front end of libh2h (the C ABI in include/h2h.h) over ctypes.
+
+The methods mirror the reference's phase entry points (main.cu:149-240):
+contraction_noHub -> contract, constructLabel_noHub -> build_labels,
+H2HDistancQuery_UsingLCA_noHub / H2HDistancQuery_noLCA_noHub -> query (batched
+over arrays).  Every call raises H2HError carrying the C status and message;
+nothing falls back to a CPU path: without the built library or a gfx950 GPU
+the constructor raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libh2h.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "h2h.h")
+
+OK = 0
+ERR = {
+    -1: "H2H_ERR_ARG", -2: "H2H_ERR_IO", -3: "H2H_ERR_INPUT", -4: "H2H_ERR_LUB", -5: "H2H_ERR_HIP",
+    -6: "H2H_ERR_OOM", -7: "H2H_ERR_STATE", -8: "H2H_ERR_NOGPU", -9: "H2H_ERR_ADMIT",
+    -10: "H2H_ERR_TIMEOUT", -11: "H2H_ERR_CANCELLED",
+}
+CONTRACT_CPU_ORDER, CONTRACT_GPU_ORDER = 0, 1
+L3, L4 = 3, 4
+PF, LL, BATCHED = 1, 2, 3
+QUERY_LCA, QUERY_NOLCA = 0, 1
+EXEC_PLAIN, EXEC_BATCHED = 0, 1
+
+
+class H2HError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{ERR.get(status, status)}: {msg}")
+        self.status = status
+
+
+class Info(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in (
+        "n", "arcs", "partition_height", "change_height", "label_entries", "pos_entries", "tree_height",
+        "levels", "top_vertices", "head", "lub_entries", "device_bytes")] + [
+        (k, ctypes.c_double) for k in ("t_partition_us", "t_contract_us", "t_tree_us", "t_label_us")]
+
+
+class ServerCfg(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int), ("max_inflight", ctypes.c_int64), ("max_request", ctypes.c_int64),
+                ("kind", ctypes.c_int), ("exec", ctypes.c_int)]
+
+
+class ReqStats(ctypes.Structure):
+    _fields_ = [("submit_us", ctypes.c_double), ("start_us", ctypes.c_double), ("done_us", ctypes.c_double),
+                ("n", ctypes.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads lib/libh2h.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise H2HError(-5, f"{LIB_PATH} is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        dp = ctypes.POINTER(ctypes.c_double)
+        sig = {
+            "h2h_create": [ctypes.POINTER(P), ctypes.c_int],
+            "h2h_destroy": [P],
+            "h2h_last_error": [P],
+            "h2h_load_files": [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int],
+            "h2h_set_graph": [P, ctypes.c_int32, ctypes.c_int64, i32p, i32p, i32p, i32p, ctypes.c_int, ctypes.c_int, i32p],
+            "h2h_partition_rank_tree": [P, i32p, i32p, dp],
+            "h2h_contract": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp],
+            "h2h_build_labels": [P, ctypes.c_int, ctypes.c_int, dp],
+            "h2h_labels_upload": [P, ctypes.c_int32, i64p, i32p, i64p, i32p, i32p, i32p, i32p],
+            "h2h_query": [P, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int64, dp],
+            "h2h_query_device": [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64, P],
+            "h2h_get_info": [P, ctypes.POINTER(Info)],
+            "h2h_get_order": [P, i32p, i32p, i32p],
+            "h2h_get_tree": [P, i32p, i32p, i32p, i32p, i32p],
+            "h2h_get_bags": [P, i64p, i32p, i32p],
+            "h2h_get_labels": [P, i64p, i32p, i64p, i32p, i32p],
+            "h2h_server_start": [P, ctypes.POINTER(ServerCfg)],
+            "h2h_submit": [P, i32p, i32p, i32p, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64)],
+            "h2h_wait": [P, ctypes.c_uint64, ctypes.c_int, ctypes.POINTER(ReqStats)],
+            "h2h_cancel": [P, ctypes.c_uint64],
+            "h2h_server_stop": [P],
+            "h2h_write_construct_info": [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        L.h2h_destroy.restype = None
+        L.h2h_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def declared_symbols():
+    """Names of the functions include/h2h.h declares."""
+    import re
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(h2h_[a-z_]+)\s*\(", text)))
+
+
+def _p32(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)) if a is not None else None
+
+
+def _p64(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)) if a is not None else None
+
+
+class H2HIndex:
+    """One device-resident H2H index (an h2h_ctx)."""
+
+    def __init__(self, device=0):
+        """device=-1 gives a host-only context (loading, validation, partition tree)."""
+        self.h = ctypes.c_void_p()
+        st = lib().h2h_create(ctypes.byref(self.h), int(device))
+        if st != OK:
+            raise H2HError(st, "h2h_create failed (no gfx950 device?)")
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().h2h_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st):
+        if st != OK:
+            raise H2HError(st, lib().h2h_last_error(self.h).decode(errors="replace"))
+
+    # ---- input ----
+    def load_files(self, edge, idf, cut, tree_height):
+        self._chk(lib().h2h_load_files(self.h, edge.encode(), idf.encode(), cut.encode(), int(tree_height)))
+
+    def set_graph(self, n, arc_u, arc_v, arc_w, index_ids, cut_lr, height_file, height_arg=None):
+        u = np.ascontiguousarray(arc_u, np.int32)
+        v = np.ascontiguousarray(arc_v, np.int32)
+        w = np.ascontiguousarray(arc_w, np.int32)
+        ids = np.ascontiguousarray(index_ids, np.int32)
+        cut = np.ascontiguousarray(np.asarray(cut_lr, np.int32).reshape(-1))
+        self._chk(lib().h2h_set_graph(self.h, int(n), len(u), _p32(u), _p32(v), _p32(w), _p32(ids),
+                                      int(height_file), int(height_arg if height_arg is not None else height_file),
+                                      _p32(cut)))
+
+    # ---- pipeline ----
+    def partition_rank_tree(self):
+        H = self.info()["partition_height"]
+        n = self.info()["n"]
+        nan = np.empty(3 * ((1 << H) - 1), np.int32)
+        elim = np.empty(n, np.int32)
+        frac = np.empty(H, np.float64)
+        self._chk(lib().h2h_partition_rank_tree(self.h, _p32(nan), _p32(elim),
+                                                frac.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return nan.reshape(-1, 3), elim, frac
+
+    def contract(self, change_height, order=CONTRACT_GPU_ORDER, threads=1):
+        us = ctypes.c_double()
+        self._chk(lib().h2h_contract(self.h, int(change_height), int(order), int(threads), ctypes.byref(us)))
+        return us.value
+
+    def build_labels(self, kind=L4, method=PF):
+        us = ctypes.c_double()
+        self._chk(lib().h2h_build_labels(self.h, int(kind), int(method), ctypes.byref(us)))
+        return us.value
+
+    def labels_upload(self, dis_hash, dis, pos_hash, pos_id, pos_pos, father, depth):
+        arrs = [np.ascontiguousarray(dis_hash, np.int64), np.ascontiguousarray(dis, np.int32),
+                np.ascontiguousarray(pos_hash, np.int64), np.ascontiguousarray(pos_id, np.int32),
+                np.ascontiguousarray(pos_pos, np.int32), np.ascontiguousarray(father, np.int32),
+                np.ascontiguousarray(depth, np.int32)]
+        self._chk(lib().h2h_labels_upload(self.h, len(arrs[5]), _p64(arrs[0]), _p32(arrs[1]), _p64(arrs[2]),
+                                          _p32(arrs[3]), _p32(arrs[4]), _p32(arrs[5]), _p32(arrs[6])))
+
+    def query(self, x, y, kind=QUERY_LCA, exec_mode=EXEC_BATCHED):
+        x = np.ascontiguousarray(x, np.int32)
+        y = np.ascontiguousarray(y, np.int32)
+        out = np.empty(len(x), np.int32)
+        us = ctypes.c_double()
+        self._chk(lib().h2h_query(self.h, int(kind), int(exec_mode), _p32(x), _p32(y), _p32(out), len(x),
+                                  ctypes.byref(us)))
+        return out, us.value
+
+    def query_device(self, x_ptr, y_ptr, out_ptr, n, kind=QUERY_LCA, exec_mode=EXEC_BATCHED, stream=None):
+        self._chk(lib().h2h_query_device(self.h, int(kind), int(exec_mode), ctypes.c_void_p(x_ptr),
+                                         ctypes.c_void_p(y_ptr), ctypes.c_void_p(out_ptr), int(n),
+                                         ctypes.c_void_p(stream) if stream else None))
+
+    # reference-named aliases (main.cu:149-188, Query.cu:108-151)
+    def contraction_noHub(self, contract_device, change_height, threads=1):
+        return self.contract(change_height, CONTRACT_GPU_ORDER if contract_device else CONTRACT_CPU_ORDER, threads)
+
+    def constructLabel_noHub(self, parallel_fine, l4=True):
+        return self.build_labels(L4 if l4 else L3, {0: PF, 1: PF, 2: LL}.get(parallel_fine, BATCHED))
+
+    def H2HDistancQuery_UsingLCA_noHub(self, x, y):
+        return self.query(x, y, QUERY_LCA)[0]
+
+    def H2HDistancQuery_noLCA_noHub(self, x, y):
+        return self.query(x, y, QUERY_NOLCA)[0]
+
+    # ---- readback ----
+    def info(self):
+        inf = Info()
+        self._chk(lib().h2h_get_info(self.h, ctypes.byref(inf)))
+        return {k: getattr(inf, k) for k, _ in Info._fields_}
+
+    def order(self):
+        n = self.info()["n"]
+        e, d, h = (np.empty(n, np.int32) for _ in range(3))
+        self._chk(lib().h2h_get_order(self.h, _p32(e), _p32(d), _p32(h)))
+        return e, d, h
+
+    def tree(self):
+        inf = self.info()
+        n = inf["n"]
+        father, depth, first, bfs = (np.empty(n, np.int32) for _ in range(4))
+        lvl = np.empty(inf["levels"] + 1, np.int32)
+        self._chk(lib().h2h_get_tree(self.h, _p32(father), _p32(depth), _p32(first), _p32(bfs), _p32(lvl)))
+        return dict(father=father, depth=depth, first_appear=first, bfs=bfs, level_off=lvl)
+
+    def bags(self):
+        inf = self.info()
+        n = inf["n"]
+        m = inf["pos_entries"] - n
+        off = np.empty(n + 1, np.int64)
+        nbr = np.empty(max(m, 1), np.int32)
+        w = np.empty(max(m, 1), np.int32)
+        self._chk(lib().h2h_get_bags(self.h, _p64(off), _p32(nbr), _p32(w)))
+        return off, nbr[:m], w[:m]
+
+    def labels(self, with_dis=True):
+        inf = self.info()
+        n = inf["n"]
+        dh = np.empty(n + 1, np.int64)
+        ph = np.empty(n + 1, np.int64)
+        pid = np.empty(inf["pos_entries"], np.int32)
+        pp = np.empty(inf["pos_entries"], np.int32)
+        dis = np.empty(inf["label_entries"], np.int32) if with_dis else None
+        self._chk(lib().h2h_get_labels(self.h, _p64(dh), _p32(dis), _p64(ph), _p32(pid), _p32(pp)))
+        return dict(dis_hash=dh, dis=dis, pos_hash=ph, pos_id=pid, pos_pos=pp)
+
+    def write_construct_info(self, path, graph_name, partition_method=1, threads=1, contract_device=1,
+                             parallel_fine=1, construct_device=1):
+        self._chk(lib().h2h_write_construct_info(self.h, path.encode(), graph_name.encode(), partition_method,
+                                                 threads, contract_device, parallel_fine, construct_device))
+
+    # ---- serving ----
+    def server_start(self, streams=4, max_inflight=1 << 20, max_request=1 << 16, kind=QUERY_LCA,
+                     exec_mode=EXEC_BATCHED):
+        cfg = ServerCfg(streams, max_inflight, max_request, kind, exec_mode)
+        self._chk(lib().h2h_server_start(self.h, ctypes.byref(cfg)))
+
+    def submit(self, x, y):
+        """Returns (ticket, out_array); out_array is filled when wait() returns."""
+        x = np.ascontiguousarray(x, np.int32)
+        y = np.ascontiguousarray(y, np.int32)
+        out = np.empty(len(x), np.int32)
+        t = ctypes.c_uint64()
+        st = lib().h2h_submit(self.h, _p32(x), _p32(y), _p32(out), len(x), ctypes.byref(t))
+        if st != OK:
+            raise H2HError(st, "h2h_submit")
+        self._keep.append((t.value, out))
+        return t.value, out
+
+    def wait(self, ticket, timeout_ms=-1):
+        s = ReqStats()
+        st = lib().h2h_wait(self.h, ctypes.c_uint64(ticket), int(timeout_ms), ctypes.byref(s))
+        if st != OK:
+            raise H2HError(st, "h2h_wait")
+        self._keep = [k for k in self._keep if k[0] != ticket]
+        return dict(submit_us=s.submit_us, start_us=s.start_us, done_us=s.done_us, n=s.n)
+
+    def cancel(self, ticket):
+        self._chk(lib().h2h_cancel(self.h, ctypes.c_uint64(ticket)))
+
+    def server_stop(self):
+        self._chk(lib().h2h_server_stop(self.h))

@@ -1,0 +1,195 @@
+/*
+ * h2h.h -- C ABI of the MI355X H2H (hierarchical 2-hop) index engine.
+ *
+ * One h2h_ctx owns one GPU's copy of the index (device memory, a HIP stream,
+ * host-side tree metadata).  Every entry point returns an h2h_status; none
+ * calls exit().  Pointers are plain host pointers unless the name says
+ * "device".  Vertex ids are 0-based int32 (the reference's internal ids,
+ * i.e. Edge-file id - 1).  Not thread-safe per context: drive one context
+ * from one host thread at a time (the serving loop has its own internal
+ * thread and is safe to call from many threads).
+ *
+ * Each declaration names the reference interface it replaces
+ * (sauccjy/GPU-H2H, file:line).
+ */
+#ifndef H2H_H_
+#define H2H_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct h2h_ctx h2h_ctx;
+typedef int h2h_status;
+
+enum {
+  H2H_OK = 0,
+  H2H_ERR_ARG = -1,       /* null pointer, flag out of range, id out of range */
+  H2H_ERR_IO = -2,        /* input file missing or malformed */
+  H2H_ERR_INPUT = -3,     /* graph rejected: disconnected, asymmetric arcs, parallel arcs, bad cut heap */
+  H2H_ERR_LUB = -4,       /* a vertex outgrew its slot capacity (GraphPartition.cu:114-118) */
+  H2H_ERR_HIP = -5,       /* HIP runtime error (message in h2h_last_error) */
+  H2H_ERR_OOM = -6,       /* device or host allocation refused */
+  H2H_ERR_STATE = -7,     /* entry point called out of order */
+  H2H_ERR_NOGPU = -8,     /* no usable gfx950 device */
+  H2H_ERR_ADMIT = -9,     /* serving: request larger than the admission limit */
+  H2H_ERR_TIMEOUT = -10,  /* serving: wait timed out */
+  H2H_ERR_CANCELLED = -11 /* serving: request cancelled or server stopped */
+};
+
+/* useGPUContract (main.cu:258): order of the partition-level eliminations. */
+enum {
+  H2H_CONTRACT_CPU_ORDER = 0, /* partition_TD_T order (Traditional_H2H.cu:354-496) */
+  H2H_CONTRACT_GPU_ORDER = 1  /* makeCHPerHeight_D_2 order (kernel_functions.cu:310-479) */
+};
+/* label kind: L3 = as shipped (ancestor loop commented out), L4 = exact. */
+enum { H2H_LABEL_L3 = 3, H2H_LABEL_L4 = 4 };
+/* parallelFine (main.cu:259) plus the batched persistent build. */
+enum { H2H_METHOD_PF = 1, H2H_METHOD_LL = 2, H2H_METHOD_BATCHED = 3 };
+/* query kind */
+enum { H2H_QUERY_LCA = 0, H2H_QUERY_NOLCA = 1 };
+/* query execution: one lane per query (reference kernels) or grouped lanes */
+enum { H2H_EXEC_PLAIN = 0, H2H_EXEC_BATCHED = 1 };
+
+typedef struct {
+  int64_t n;                 /* NodeNumber */
+  int64_t arcs;              /* arcs after self-loop removal (EdgeNumber) */
+  int64_t partition_height;  /* TreeHeight after min(file, argument) */
+  int64_t change_height;     /* ChangeToGlobalHeight */
+  int64_t label_entries;     /* H2H_Size = sum of depth(v) */
+  int64_t pos_entries;       /* sum of |X(v)| + 1 */
+  int64_t tree_height;       /* CHTreeHeight = max depth */
+  int64_t levels;            /* BFS levels of the tree */
+  int64_t top_vertices;      /* vertices eliminated by the global top phase */
+  int64_t head;              /* root vertex */
+  int64_t lub_entries;       /* slot entries the contraction reserved */
+  int64_t device_bytes;      /* device memory held by the index */
+  double t_partition_us;     /* makeAdjcentNode equivalent (host) */
+  double t_contract_us;      /* partition levels + top phase (device events) */
+  double t_tree_us;          /* father/children/Euler/label layout (host) */
+  double t_label_us;         /* last label build (device events) */
+} h2h_info;
+
+/* ---- lifecycle (Graph::Graph + startGPU, Graph_Fundamental.cu:4-95, tanslate.cu:7-10) ----
+ * device = -1 creates a host-only context: inputs load and validate, every
+ * device entry point returns H2H_ERR_NOGPU. */
+h2h_status h2h_create(h2h_ctx** out, int device);
+void h2h_destroy(h2h_ctx* ctx);
+const char* h2h_last_error(const h2h_ctx* ctx);
+
+/* ---- input ---- */
+/* Graph::Graph file loader (Graph_Fundamental.cu:4-95): Edge file "N M" then
+ * "a u v w" (1-based), ID file "N" then "idx x y id0", Cut file "H" then heap
+ * lines "h l r".  tree_height is the PartitionHeight argument. */
+h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_file,
+                          const char* cut_file, int tree_height);
+/* The same inputs from memory: m arcs (0-based, both directions listed),
+ * index_ids[n] (the ID array: index -> vertex), cut_lr[2*(2^H-1)] heap of
+ * [l, r) ranges with H = min(height_file, height_arg). */
+h2h_status h2h_set_graph(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_u,
+                         const int32_t* arc_v, const int32_t* arc_w, const int32_t* index_ids,
+                         int height_file, int height_arg, const int32_t* cut_lr);
+
+/* ---- partition rank tree (makeAdjcentNode, GraphPartition.cu:6-136) ----
+ * Host-only (works on a device -1 context).  nan_hash[3*(2^H-1)] receives
+ * NANHash {first slot, present vertices, interior vertices} per heap node,
+ * elim[n] the partition-order record (interior vertices of each partition in
+ * ascending id, levels H..1, partitions from the highest heap index down),
+ * layer_fraction[H] the cumulative interior fraction after levels H..1
+ * (the <Method>PartitionRate.csv row); any pointer may be NULL. */
+h2h_status h2h_partition_rank_tree(h2h_ctx* ctx, int32_t* nan_hash, int32_t* elim,
+                                   double* layer_fraction);
+
+/* ---- contraction (contraction_noHub, main.cu:149-158) ----
+ * makeAdjcentNode (GraphPartition.cu:6-136), the partition levels
+ * H..change_height+1 on the GPU (makeCHPerHeight_2, kernel_functions.cu:520-553,
+ * or partition_TD_T's order), the global top phase (CHUsingHeap_2,
+ * Traditional_H2H.cu:216-351) on the GPU, and the tree (beforeH2H_noHub,
+ * TreeStructure.cu:41-262).  thread_num reproduces partition_TD_T's
+ * per-thread chunking of the elimination record (it does not change the tree).
+ * usec: device time of the contraction kernels. */
+h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int thread_num,
+                        double* usec);
+
+/* ---- label construction (constructLabel_noHub, main.cu:160-188;
+ * makeH2HLabel_noCommunication_noHub kernel_functions.cuh:40-49 and
+ * makeH2HLabel_noCommunication_noHub_3 kernel_functions.cuh:64-74) ----
+ * Builds dis[v][0..depth(v)) for every vertex on the device; the index stays
+ * resident.  usec: device time (hipEvent) of the build kernels. */
+h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* usec);
+
+/* Bring-your-own index: the arrays mallocH2HLabel_noHub produces
+ * (H2HConsturction.cu:618-722): dis_hash[n+1], dis[dis_hash[n]] (bag weights at
+ * bag positions, 0 at depth-1, anything elsewhere), pos_hash[n+1],
+ * pos_id/pos_pos[pos_hash[n]] (sorted by position, self last), father[n]
+ * (-1 at the root) and depth[n] (root 1).  Replaces the contraction; follow
+ * with h2h_build_labels. */
+h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, const int32_t* dis,
+                             const int64_t* pos_hash, const int32_t* pos_id,
+                             const int32_t* pos_pos, const int32_t* father, const int32_t* depth);
+
+/* ---- queries (H2HQuery_UsingLCA_noHub kernel_functions.cuh:95-111,
+ * H2HQuery_noLCA_noHub kernel_functions.cuh:114-128) ----
+ * Host buffers x, y, out of length n.  x == y gives 0.  usec: device time. */
+h2h_status h2h_query(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const int32_t* y,
+                     int32_t* out, int64_t n, double* usec);
+/* Device buffers, enqueued on `stream` (a hipStream_t, or NULL for the
+ * context's stream); returns without synchronising. */
+h2h_status h2h_query_device(h2h_ctx* ctx, int kind, int exec, const int32_t* d_x,
+                            const int32_t* d_y, int32_t* d_out, int64_t n, void* stream);
+
+/* ---- index readback (reference layout) ---- */
+h2h_status h2h_get_info(const h2h_ctx* ctx, h2h_info* out);
+/* nonAdjcentNode after contraction: elimination order ids, and the (degree,
+ * height) key each vertex had when it was picked; any pointer may be NULL. */
+h2h_status h2h_get_order(const h2h_ctx* ctx, int32_t* elim, int32_t* rec_degree,
+                         int32_t* rec_height);
+/* father[n], depth[n] (H2H_TreeHeight_Hash), first_appear[n] (Euler tour),
+ * bfs[n] (TreeBFS), level_off[levels+1] (TreeHash); any pointer may be NULL. */
+h2h_status h2h_get_tree(const h2h_ctx* ctx, int32_t* father, int32_t* depth,
+                        int32_t* first_appear, int32_t* bfs, int32_t* level_off);
+/* Bags X(v) with shortcut weights, CSR sorted by neighbour id. */
+h2h_status h2h_get_bags(const h2h_ctx* ctx, int64_t* off, int32_t* nbr, int32_t* w);
+/* H2H_dis_hash, H2H_dis (copied from the device), H2H_pos_hash, H2H_pos_ID,
+ * H2H_pos_POS; any pointer may be NULL. */
+h2h_status h2h_get_labels(h2h_ctx* ctx, int64_t* dis_hash, int32_t* dis, int64_t* pos_hash,
+                          int32_t* pos_id, int32_t* pos_pos);
+
+/* ---- serving: continuous batching over the resident index ---- */
+typedef struct {
+  int streams;             /* concurrent device lanes (HIP streams), 1..8 */
+  int64_t max_inflight;    /* admission limit: queries on the device at once */
+  int64_t max_request;     /* largest single request accepted (H2H_ERR_ADMIT above) */
+  int kind;                /* H2H_QUERY_LCA or H2H_QUERY_NOLCA */
+  int exec;                /* H2H_EXEC_PLAIN or H2H_EXEC_BATCHED */
+} h2h_server_cfg;
+
+typedef struct {
+  double submit_us;        /* host clock at submit (since server start) */
+  double start_us;         /* admitted onto a device lane */
+  double done_us;          /* results copied back */
+  int64_t n;
+} h2h_req_stats;
+
+h2h_status h2h_server_start(h2h_ctx* ctx, const h2h_server_cfg* cfg);
+/* Copies x, y (n queries) into the server; results are written to out when the
+ * request completes (out must stay valid until h2h_wait returns). */
+h2h_status h2h_submit(h2h_ctx* ctx, const int32_t* x, const int32_t* y, int32_t* out, int64_t n,
+                      uint64_t* ticket);
+/* timeout_ms < 0 waits forever. */
+h2h_status h2h_wait(h2h_ctx* ctx, uint64_t ticket, int timeout_ms, h2h_req_stats* stats);
+h2h_status h2h_cancel(h2h_ctx* ctx, uint64_t ticket);
+h2h_status h2h_server_stop(h2h_ctx* ctx);
+
+/* ---- reporting (printConstructInfo, Graph_Fundamental.cu:132-170) ----
+ * Appends one row with the reference's 31-column header to a CSV file. */
+h2h_status h2h_write_construct_info(const h2h_ctx* ctx, const char* path, const char* graph_name,
+                                    int partition_method, int thread_num, int contract_device,
+                                    int parallel_fine, int construct_device);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H2H_H_ */

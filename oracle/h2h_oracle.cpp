@@ -1039,6 +1039,12 @@ int orc_contract(void* h, int mode, int change_height) {
   return 0;
 }
 
+// makeAdjcentNode alone (the partition rank tree before any contraction).
+int orc_partition(void* h) {
+  make_adjcent_node((Oracle*)h);
+  return 0;
+}
+
 int orc_labels_alloc(void* h) {
   malloc_labels((Oracle*)h);
   return 0;

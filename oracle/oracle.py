@@ -43,6 +43,7 @@ def lib():
         L.orc_error.argtypes = [P]
         L.orc_contract.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.orc_labels_alloc.argtypes = [P]
+        L.orc_partition.argtypes = [P]
         L.orc_labels_build.restype = ctypes.c_longlong
         L.orc_labels_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_query.restype = ctypes.c_longlong
@@ -107,6 +108,11 @@ class OracleIndex:
 
     def contract(self, mode, change_height):
         self._check(lib().orc_contract(self.h, int(mode), int(change_height)))
+
+    def partition(self):
+        """makeAdjcentNode only: NANHash, the partition-order record, layer fractions."""
+        self._check(lib().orc_partition(self.h))
+        return self.nan_hash(), self.elimination_order(), self.layer_fraction()
 
     def labels(self, method=PF, l4=True, threads=None):
         L = lib()

@@ -1,0 +1,152 @@
+"""GPU path vs the oracle, bit-exact: elimination record, tree, bags, label
+layout, L3 and L4 labels for every build method, and query answers (both
+query kinds, plain and batched); L4 answers also vs Dijkstra."""
+import numpy as np
+import pytest
+
+import cases
+import h2h
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+METHODS = (h2h.PF, h2h.LL, h2h.BATCHED)
+
+
+def _contract_both(case, C, order, threads=2):
+    o = case.oracle(threads)
+    o.contract(O.CPU if order == h2h.CONTRACT_CPU_ORDER else O.GPU_FAST, C)
+    p = h2h.H2HIndex()
+    case.product(p)
+    p.contract(C, order, threads)
+    return o, p
+
+
+def _compare_structure(o, p, exact_record):
+    e, _, hgt = p.order()
+    if exact_record:
+        np.testing.assert_array_equal(e, o.elimination_order())
+        np.testing.assert_array_equal(hgt, o.recorded_height())
+    t = p.tree()
+    np.testing.assert_array_equal(t["father"], o.father())
+    np.testing.assert_array_equal(t["depth"], o.depth())
+    np.testing.assert_array_equal(t["first_appear"], o.first_appear())
+    np.testing.assert_array_equal(t["bfs"], o.tree_bfs())
+    np.testing.assert_array_equal(t["level_off"], o.tree_hash())
+    off, nbr, w = p.bags()
+    ooff, onbr, ow = o.bags()
+    np.testing.assert_array_equal(off, ooff)
+    np.testing.assert_array_equal(nbr, onbr)
+    np.testing.assert_array_equal(w, ow)
+
+
+def _compare_labels(o, p, kind, method):
+    o.labels(O.SERIAL, l4=(kind == h2h.L4))
+    p.build_labels(kind, method)
+    lab = p.labels()
+    np.testing.assert_array_equal(lab["dis_hash"], o.dis_hash())
+    np.testing.assert_array_equal(lab["pos_hash"], o.pos_hash())
+    np.testing.assert_array_equal(lab["pos_id"], o.pos_id())
+    np.testing.assert_array_equal(lab["pos_pos"], o.pos_pos())
+    np.testing.assert_array_equal(lab["dis"], o.dis())
+
+
+def _compare_queries(o, p, x, y, kinds=(h2h.QUERY_LCA, h2h.QUERY_NOLCA)):
+    for kind in kinds:
+        ref, _ = o.query(kind, x, y, threads=4)
+        for ex in (h2h.EXEC_PLAIN, h2h.EXEC_BATCHED):
+            got, _ = p.query(x, y, kind, ex)
+            np.testing.assert_array_equal(got, ref, err_msg=f"kind={kind} exec={ex}")
+
+
+@pytest.mark.parametrize("method_name", sorted(cases.REF13_METHODS))
+@pytest.mark.parametrize("order", [h2h.CONTRACT_CPU_ORDER, h2h.CONTRACT_GPU_ORDER])
+def test_ref13_fixture(method_name, order):
+    H = cases.REF13_METHODS[method_name]
+    edge, idf, cut = cases.ref13_paths(method_name)
+    n = 13
+    xs, ys = np.meshgrid(np.arange(n), np.arange(n))
+    x = xs.ravel().astype(np.int32)
+    y = ys.ravel().astype(np.int32)
+    for C in range(1, H + 1):
+        o = O.OracleIndex.from_files(edge, idf, cut, H, threads=3)
+        o.contract(O.CPU if order == 0 else O.GPU_FAST, C)
+        with h2h.H2HIndex() as p:
+            p.load_files(edge, idf, cut, H)
+            p.contract(C, order, 3)
+            _compare_structure(o, p, exact_record=True)
+            for kind in (h2h.L3, h2h.L4):
+                for m in METHODS:
+                    _compare_labels(o, p, kind, m)
+                    kinds = (h2h.QUERY_NOLCA,) if kind == h2h.L3 else (h2h.QUERY_LCA, h2h.QUERY_NOLCA)
+                    _compare_queries(o, p, x, y, kinds)
+            D = np.array([o.dijkstra(s) for s in range(n)])
+            got, _ = p.query(x, y, h2h.QUERY_LCA)
+            np.testing.assert_array_equal(got, D[x, y])
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_small_random(seed):
+    case = cases.small_random(seed)
+    rng = np.random.default_rng(seed)
+    C = int(rng.integers(1, case.H + 1))
+    order = int(rng.integers(0, 2))
+    o, p = _contract_both(case, C, order, threads=int(rng.integers(1, 5)))
+    with p:
+        _compare_structure(o, p, exact_record=True)
+        x = rng.integers(0, case.n, 4000).astype(np.int32)
+        y = rng.integers(0, case.n, 4000).astype(np.int32)
+        for kind in (h2h.L3, h2h.L4):
+            _compare_labels(o, p, kind, METHODS[seed % 3])
+            _compare_queries(o, p, x, y, (h2h.QUERY_NOLCA,) if kind == h2h.L3 else (h2h.QUERY_LCA, h2h.QUERY_NOLCA))
+        D = case.dijkstra_rows(x[:20])
+        for i in range(20):
+            got, _ = p.query(np.full(case.n, x[i], np.int32), np.arange(case.n, dtype=np.int32), h2h.QUERY_LCA)
+            np.testing.assert_array_equal(got, D[i])
+
+
+@pytest.mark.parametrize("order", [h2h.CONTRACT_CPU_ORDER, h2h.CONTRACT_GPU_ORDER])
+def test_grid64(order):
+    case = cases.grid64()
+    o, p = _contract_both(case, 1, order, threads=4)
+    with p:
+        _compare_structure(o, p, exact_record=True)
+        rng = np.random.default_rng(7)
+        x = rng.integers(0, case.n, 100000).astype(np.int32)
+        y = rng.integers(0, case.n, 100000).astype(np.int32)
+        for kind in (h2h.L3, h2h.L4):
+            for m in METHODS:
+                _compare_labels(o, p, kind, m)
+            _compare_queries(o, p, x, y, (h2h.QUERY_NOLCA,) if kind == h2h.L3 else (h2h.QUERY_LCA, h2h.QUERY_NOLCA))
+        src = rng.integers(0, case.n, 16)
+        D = case.dijkstra_rows(src)
+        for i, s in enumerate(src):
+            got, _ = p.query(np.full(case.n, s, np.int32), np.arange(case.n, dtype=np.int32), h2h.QUERY_LCA)
+            np.testing.assert_array_equal(got, D[i])
+
+
+@pytest.mark.parametrize("C", [1, 3])
+def test_delaunay_30k(C):
+    case = cases.delaunay(30000, 2.6, 5, 7)
+    o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=8)
+    with p:
+        _compare_structure(o, p, exact_record=True)
+        _compare_labels(o, p, h2h.L4, h2h.PF)
+        rng = np.random.default_rng(11)
+        x = rng.integers(0, case.n, 200000).astype(np.int32)
+        y = rng.integers(0, case.n, 200000).astype(np.int32)
+        _compare_queries(o, p, x, y)
+
+
+def test_labels_upload_roundtrip():
+    """Bring-your-own index: the rows mallocH2HLabel_noHub leaves (bag weights,
+    0 on the diagonal) plus pos/father/depth, built on the GPU."""
+    case = cases.grid64()
+    o = case.oracle(4)
+    o.contract(O.GPU_FAST, 1)
+    assert O.lib().orc_labels_alloc(o.h) == 0
+    with h2h.H2HIndex() as p:
+        p.labels_upload(o.dis_hash(), o.dis(), o.pos_hash(), o.pos_id(), o.pos_pos(), o.father(), o.depth())
+        for kind in (h2h.L3, h2h.L4):
+            p.build_labels(kind, h2h.PF)
+            o.labels(O.SERIAL, l4=(kind == h2h.L4))
+            np.testing.assert_array_equal(p.labels()["dis"], o.dis())

@@ -1,0 +1,155 @@
+"""Host-side stages on a host-only context (no GPU): input loading and
+validation, status codes for bad inputs, and the partition rank tree
+against the oracle's makeAdjcentNode restatement."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import h2h
+import h2h_gen as G
+
+
+def _host():
+    return h2h.H2HIndex(device=-1)
+
+
+@pytest.mark.parametrize("method_name", sorted(cases.REF13_METHODS))
+def test_partition_rank_tree_ref13(method_name):
+    H = cases.REF13_METHODS[method_name]
+    edge, idf, cut = cases.ref13_paths(method_name)
+    o = cases.O.OracleIndex.from_files(edge, idf, cut, H)
+    onan, oelim, ofrac = o.partition()
+    with _host() as p:
+        p.load_files(edge, idf, cut, H)
+        nan, elim, frac = p.partition_rank_tree()
+    np.testing.assert_array_equal(nan, onan)
+    np.testing.assert_array_equal(elim, oelim)
+    np.testing.assert_allclose(frac, ofrac)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_partition_rank_tree_random(seed):
+    case = cases.small_random(seed) if seed < 10 else cases.grid64()
+    o = case.oracle()
+    onan, oelim, ofrac = o.partition()
+    with _host() as p:
+        case.product(p)
+        nan, elim, frac = p.partition_rank_tree()
+    np.testing.assert_array_equal(nan, onan)
+    np.testing.assert_array_equal(elim, oelim)
+    np.testing.assert_allclose(frac, ofrac)
+
+
+def _tri():
+    # path 0-1-2 plus 2-0, weights 1
+    u = np.array([0, 1, 1, 2, 2, 0], np.int32)
+    v = np.array([1, 0, 2, 1, 0, 2], np.int32)
+    return u, v, np.ones(6, np.int32)
+
+
+def _cut1(n):
+    return np.array([[0, n]], np.int32)
+
+
+def test_error_disconnected():
+    u = np.array([0, 1], np.int32)
+    v = np.array([1, 0], np.int32)
+    with _host() as p, pytest.raises(h2h.H2HError) as e:
+        p.set_graph(3, u, v, np.ones(2, np.int32), np.arange(3), _cut1(3), 1)
+    assert e.value.status == -3
+
+
+def test_error_asymmetric_and_parallel():
+    u, v, w = _tri()
+    with _host() as p:
+        with pytest.raises(h2h.H2HError) as e:
+            p.set_graph(3, u[:-1], v[:-1], w[:-1], np.arange(3), _cut1(3), 1)
+        assert e.value.status == -3
+        with pytest.raises(h2h.H2HError) as e:
+            p.set_graph(3, np.r_[u, 0], np.r_[v, 1], np.r_[w, 5], np.arange(3), _cut1(3), 1)
+        assert e.value.status == -3
+
+
+def test_error_ids_and_cut():
+    u, v, w = _tri()
+    with _host() as p:
+        with pytest.raises(h2h.H2HError) as e:
+            p.set_graph(3, u, v, w, np.array([0, 0, 1]), _cut1(3), 1)
+        assert e.value.status == -3
+        with pytest.raises(h2h.H2HError) as e:
+            p.set_graph(3, u, v, w, np.arange(3), np.array([[0, 4]]), 1)
+        assert e.value.status == -3
+        with pytest.raises(h2h.H2HError) as e:
+            p.set_graph(3, np.r_[u, 7], np.r_[v, 0], np.r_[w, 1], np.arange(3), _cut1(3), 1)
+        assert e.value.status == -3
+        with pytest.raises(h2h.H2HError) as e:
+            p.set_graph(3, u, v, -w, np.arange(3), _cut1(3), 1)
+        assert e.value.status == -3
+
+
+def test_self_loops_are_dropped():
+    u, v, w = _tri()
+    with _host() as p:
+        p.set_graph(3, np.r_[u, 1], np.r_[v, 1], np.r_[w, 9], np.arange(3), _cut1(3), 1)
+        assert p.info()["arcs"] == 6
+
+
+def test_cut_not_covering_every_vertex():
+    u, v, w = _tri()
+    with _host() as p:
+        p.set_graph(3, u, v, w, np.arange(3), np.array([[0, 2]]), 1)
+        with pytest.raises(h2h.H2HError) as e:
+            p.partition_rank_tree()
+        assert e.value.status == -3
+
+
+def test_device_calls_on_host_context(tmp_path):
+    u, v, w = _tri()
+    with _host() as p:
+        p.set_graph(3, u, v, w, np.arange(3), _cut1(3), 1)
+        for call in (lambda: p.contract(1), lambda: p.build_labels(), lambda: p.query([0], [1]),
+                     lambda: p.server_start()):
+            with pytest.raises(h2h.H2HError) as e:
+                call()
+            assert e.value.status == -8
+
+
+def test_file_errors(tmp_path):
+    edge, idf, cut = cases.ref13_paths("Latitude")
+    with _host() as p:
+        with pytest.raises(h2h.H2HError) as e:
+            p.load_files(str(tmp_path / "missing.txt"), idf, cut, 4)
+        assert e.value.status == -2
+        bad = tmp_path / "Edge.txt"
+        bad.write_text("13 44\na 1 2 4\n")
+        with pytest.raises(h2h.H2HError) as e:
+            p.load_files(str(bad), idf, cut, 4)
+        assert e.value.status == -2
+        p.load_files(edge, idf, cut, 9)          # height argument above the file's: min(4, 9)
+        assert p.info()["partition_height"] == 4
+        p.load_files(edge, idf, cut, 2)
+        assert p.info()["partition_height"] == 2
+
+
+def test_state_errors():
+    with _host() as p:
+        with pytest.raises(h2h.H2HError) as e:
+            p.partition_rank_tree()
+        assert e.value.status == -7
+
+
+def test_generated_files_roundtrip(tmp_path):
+    g, ids, cut, H = G.make_workload("grid64")
+    G.write_files(g, ids, cut, H, str(tmp_path), "grid64")
+    with _host() as p:
+        p.load_files(str(tmp_path / "Graph/USA/grid64Edge.txt"), str(tmp_path / "Partition/grid64/LatitudeID.txt"),
+                     str(tmp_path / "Partition/grid64/LatitudeCut.txt"), H)
+        nan, elim, _ = p.partition_rank_tree()
+    c = cases.grid64()
+    with _host() as p:
+        c.product(p)
+        nan2, elim2, _ = p.partition_rank_tree()
+    np.testing.assert_array_equal(nan, nan2)
+    np.testing.assert_array_equal(elim, elim2)
