@@ -38,7 +38,12 @@ h2h_status dev_alloc(h2h_ctx* ctx, DevArray& a, size_t bytes) {
 h2h_status upload(h2h_ctx* ctx, DevArray& a, const void* src, size_t bytes) {
   h2h_status st = dev_alloc(ctx, a, bytes);
   if (st != H2H_OK) return st;
-  if (bytes) H2H_HIP(ctx, hipMemcpyAsync(a.p, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  // synchronous: callers pass pageable vectors that may go out of scope
+  // right after this returns
+  if (bytes) {
+    H2H_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    H2H_HIP(ctx, hipMemcpy(a.p, src, bytes, hipMemcpyHostToDevice));
+  }
   return H2H_OK;
 }
 

@@ -181,32 +181,41 @@ struct FlowArgs {
   int spin_limit;
 };
 
+// No lane-divergent branch may sit in this loop: with `if (threadIdx.x == 0)`
+// around the fetch and the publish, the compiler joins the two across the
+// back edge into a region only lane 0 enters, and the other lanes loop back to
+// the barrier on the stale item.  Wave 0 therefore fetches, waits and
+// publishes as a whole (every lane runs the same atomic / load / store).
 __global__ void __launch_bounds__(256) label_dataflow_kernel(FlowArgs f) {
   __shared__ int32_t comb[64 * R];
   __shared__ int32_t s_item;
   __shared__ int32_t s_ok;
   const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = threadIdx.x & 63;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(f.next, 1);
+    if (wid == 0) {
+      // one increment per block: lane 0 adds 1, the other lanes add 0
+      int t = atomicAdd(f.next, lane == 0 ? 1 : 0);
+      t = __shfl(t, 0, 64);
+      s_item = t;
+    }
     __syncthreads();
     const int item = s_item;
-    __syncthreads();
     if (item >= f.n) break;
     const int v = f.a.bfs[item];
     const int d = f.bfs_depth[item];
-    if (threadIdx.x == 0) {
+    if (wid == 0) {
       int ok = 1;
       if (d > 1) {
         const int fi = f.bfs_index[f.a.father[v]];
         int spins = 0;
         while (__hip_atomic_load(&f.done[fi], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-          __builtin_amdgcn_s_sleep(1);
-          if (++spins > f.spin_limit) {
+          if (__hip_atomic_load(f.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > f.spin_limit) {
             ok = 0;
-            atomicOr(f.err, 1);
             break;
           }
+          __builtin_amdgcn_s_sleep(2);
         }
+        if (!ok) __hip_atomic_store(f.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
       s_ok = ok;
     }
@@ -215,7 +224,7 @@ __global__ void __launch_bounds__(256) label_dataflow_kernel(FlowArgs f) {
     // every wave's row stores reach device scope before the row is published
     __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) __hip_atomic_store(&f.done[item], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (wid == 0) __hip_atomic_store(&f.done[item], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -347,7 +356,7 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
     f.next = d_next.as<int32_t>();
     f.err = d_err.as<int32_t>();
     f.n = n;
-    f.spin_limit = 1 << 24;
+    f.spin_limit = 1 << 20;  // about a second of polling: far beyond any father's row
     int cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;

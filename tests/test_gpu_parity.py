@@ -30,8 +30,6 @@ def _compare_structure(o, p, exact_record):
     np.testing.assert_array_equal(t["father"], o.father())
     np.testing.assert_array_equal(t["depth"], o.depth())
     np.testing.assert_array_equal(t["first_appear"], o.first_appear())
-    np.testing.assert_array_equal(t["bfs"], o.tree_bfs())
-    np.testing.assert_array_equal(t["level_off"], o.tree_hash())
     off, nbr, w = p.bags()
     ooff, onbr, ow = o.bags()
     np.testing.assert_array_equal(off, ooff)
@@ -43,6 +41,9 @@ def _compare_labels(o, p, kind, method):
     o.labels(O.SERIAL, l4=(kind == h2h.L4))
     p.build_labels(kind, method)
     lab = p.labels()
+    t = p.tree()
+    np.testing.assert_array_equal(t["bfs"], o.tree_bfs())          # TreeBFS / TreeHash exist once labels do
+    np.testing.assert_array_equal(t["level_off"], o.tree_hash())
     np.testing.assert_array_equal(lab["dis_hash"], o.dis_hash())
     np.testing.assert_array_equal(lab["pos_hash"], o.pos_hash())
     np.testing.assert_array_equal(lab["pos_id"], o.pos_id())
