@@ -1,0 +1,273 @@
+"""Benchmark: H2H label build + queries on one MI355X.
+
+Workload (the road-like NY-scale configuration): 264,346 seeded uniform
+points in the unit square, Delaunay edges pruned to average degree 2.8
+(connected), int32 weights round(1e4 * euclid * U[1, 1.5]), recursive
+coordinate bisection of height 10, GPU-order contraction with
+ChangeToGlobalHeight 1, L4 labels.
+
+A step is one full label build (every BFS level of the elimination tree) of
+the resident index: the batched plan replayed from its hipGraph.  `value` is
+label entries written per second (sum of depth(v) / step time).  Beside it the
+line carries the contraction time, query throughput on 1e6 uniform pairs
+(plain = one lane per query, batched = grouped lanes), 4096-query batch
+latency (direct and through the serving loop), the label kernel's roofline and
+the CPU oracle timed on this host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpu-h2h_amd"))
+
+import numpy as np  # noqa: E402
+
+import h2h  # noqa: E402
+import h2h_gen as G  # noqa: E402
+
+METRIC = "H2H label build sec + queries/sec & p99 latency, 1 MI355X, plain vs batched"
+HBM_PEAK_GBS = 8000.0
+
+
+def label_bytes(lab):
+    """Algorithmic bytes of one build (L3 and L4): per node 24 B of offsets and
+    its row write, per bag entry 16 B of pos/offset reads plus the ancestor
+    prefix read (4 p_u); L4 adds the ancestor-id row and one 4 B gather per
+    column below the entry."""
+    dh, ph, pp = lab["dis_hash"], lab["pos_hash"], lab["pos_pos"]
+    n = len(dh) - 1
+    depth = np.diff(dh)
+    cnt = np.diff(ph) - 1                       # bag entries (self excluded)
+    last = ph[1:] - 1                           # self entries
+    mask = np.ones(len(pp), bool)
+    mask[last] = False
+    p = pp[mask].astype(np.int64)
+    owner_d = np.repeat(depth, cnt)
+    b3 = int(np.sum(24 + 4 * depth) + np.sum(16 + 4 * p))
+    b4 = b3 + int(np.sum(4 * depth) + np.sum(4 * np.maximum(0, owner_d - 2 - p)))
+    return b3, b4, n
+
+
+def calibration_copy(nbytes=256 << 20):
+    """One device-to-device copy of a known size: the reference point that
+    scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        return
+    hip.hipMemset(a, 1, ctypes.c_size_t(nbytes))
+    hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def cpu_cores():
+    try:
+        return min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return min(16, os.cpu_count() or 1)
+
+
+def served_latency(p, n, rate_qps, batches, size, seed):
+    """Poisson arrivals of `size`-query batches at `rate_qps` into the serving loop."""
+    rng = np.random.default_rng(seed)
+    reqs = [(rng.integers(0, n, size).astype(np.int32), rng.integers(0, n, size).astype(np.int32))
+            for _ in range(batches)]
+    gaps = rng.exponential(size / rate_qps, batches)
+    p.server_start(streams=4, max_inflight=8 * size, max_request=size, kind=h2h.QUERY_LCA,
+                   exec_mode=h2h.EXEC_BATCHED)
+    tickets = []
+    lat = []
+    try:
+        done = threading.Event()
+
+        def waiter():
+            i = 0
+            while i < batches:
+                if i >= len(tickets):
+                    time.sleep(0.0001)
+                    continue
+                st = p.wait(tickets[i])
+                lat.append((st["done_us"] - st["submit_us"]) / 1000.0)
+                i += 1
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_next = time.perf_counter()
+        for k in range(batches):
+            t_next += gaps[k]
+            while time.perf_counter() < t_next:
+                pass
+            t, _ = p.submit(*reqs[k])
+            tickets.append(t)
+        th.join()
+    finally:
+        p.server_stop()
+    return np.array(lat)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="ny264k")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-build", action="store_true",
+                    help="label build steps and the profiled pass only (counter runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the index is built and served on a single MI355X; "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    g, ids, cut, H = G.make_workload(args.workload)
+    u, v, w = g.arcs()
+    p = h2h.H2HIndex(0)
+    p.set_graph(g.n, u, v, w, ids, cut, H)
+    t0 = time.perf_counter()
+    contract_dev_us = p.contract(1, h2h.CONTRACT_GPU_ORDER, cpu_cores())
+    contract_wall = time.perf_counter() - t0
+    info = p.info()
+
+    # ---- label build steps ----
+    p.build_labels(h2h.L4, h2h.BATCHED)          # plans + graph capture
+    for _ in range(args.warmup):
+        p.build_labels(h2h.L4, h2h.BATCHED)
+    dev = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.append(p.build_labels(h2h.L4, h2h.BATCHED))   # each call ends with a device synchronize
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / args.steps
+    entries = info["label_entries"]
+    value = entries / (ms_per_step / 1000.0)
+    if args.only_build:
+        calibration_copy()
+        kern_us, launches = p.build_labels_profiled(h2h.L4, h2h.BATCHED)
+        print(json.dumps({"label_build_ms": ms_per_step, "kernel_us": kern_us, "launches": launches}))
+        return
+    plain_build_us = [p.build_labels(h2h.L4, h2h.PF) for _ in range(3)]
+    ll_build_us = [p.build_labels(h2h.L4, h2h.LL) for _ in range(3)]
+    l3_build_us = [p.build_labels(h2h.L3, h2h.BATCHED) for _ in range(3)]
+
+    # ---- roofline of the label kernel (per-launch events, no graph) ----
+    kern_us, launches = p.build_labels_profiled(h2h.L4, h2h.BATCHED)
+    lab = p.labels(with_dis=False)
+    b3, b4, _ = label_bytes(lab)
+    achieved = b4 / (kern_us * 1e-6) / 1e9
+
+    # ---- queries ----
+    p.build_labels(h2h.L4, h2h.BATCHED)
+    rng = np.random.default_rng(12345)
+    nq = 1000000
+    x = rng.integers(0, g.n, nq).astype(np.int32)
+    y = rng.integers(0, g.n, nq).astype(np.int32)
+    q = {}
+    ref = None
+    for kind, kname in ((h2h.QUERY_LCA, "lca"), (h2h.QUERY_NOLCA, "nolca")):
+        for ex, ename in ((h2h.EXEC_PLAIN, "plain"), (h2h.EXEC_BATCHED, "batched")):
+            us = []
+            for _ in range(5):
+                r, t = p.query(x, y, kind, ex)
+                us.append(t)
+            if ref is None:
+                ref = r
+            if not np.array_equal(r, ref):
+                raise SystemExit(f"query variants disagree ({kname}/{ename})")
+            q[f"{kname}_{ename}_qps"] = nq / (min(us) * 1e-6)
+    # spot check against Dijkstra
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import dijkstra
+    A = csr_matrix((w.astype(np.float64), (u, v)), shape=(g.n, g.n))
+    src = x[:3]
+    D = dijkstra(A, indices=src).astype(np.int64)
+    for i, s in enumerate(src):
+        r, _ = p.query(np.full(g.n, s, np.int32), np.arange(g.n, dtype=np.int32), h2h.QUERY_LCA)
+        if not np.array_equal(r, D[i]):
+            raise SystemExit("GPU distances differ from Dijkstra")
+
+    # ---- 4096-query batch latency ----
+    lat_direct = []
+    for k in range(200):
+        xb = x[k * 4096:(k + 1) * 4096]
+        yb = y[k * 4096:(k + 1) * 4096]
+        t0 = time.perf_counter()
+        p.query(xb, yb, h2h.QUERY_LCA, h2h.EXEC_BATCHED)
+        lat_direct.append((time.perf_counter() - t0) * 1000.0)
+    lat_direct = np.array(lat_direct[10:])
+    rate = 0.5 * 4096 / (np.median(lat_direct) / 1000.0)
+    lat_served = served_latency(p, g.n, rate, 400, 4096, 7)
+
+    # ---- CPU baseline: the oracle on this host's cores ----
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        T = cpu_cores()
+        o = O.OracleIndex(g.n, u, v, w, ids, cut, H, threads=T)
+        o.contract(O.GPU_FAST, 1)
+        label_us = o.labels(O.PF, l4=True, threads=T)
+        cq, cq_us = o.query(O.Q_NOLCA, x, y, threads=T)
+        if not np.array_equal(cq, ref):
+            raise SystemExit("CPU oracle and GPU answers differ")
+        cpu = {"value": entries / (label_us * 1e-6), "unit": "label entries/s", "cores": T, "kind": "port",
+               "sample": f"full L4 label build of the same {g.n}-vertex index on {T} threads "
+                         "(makeH2HLabel_noHub_multiThred rule with the L4 loop); "
+                         f"label_build_s={label_us * 1e-6:.4f}",
+               "label_build_s": label_us * 1e-6,
+               "query_nolca_qps": nq / (cq_us * 1e-6)}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "label entries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded road-like Delaunay graph)",
+        "config": {"workload": f"{args.workload}: {g.n} vertices, {g.m} edges, avg degree "
+                               f"{2 * g.m / g.n:.2f}, coordinate bisection H={H}, ChangeToGlobalHeight=1, "
+                               "GPU-order contraction, L4 labels, batched build",
+                   "label_entries": entries, "tree_height": info["tree_height"], "levels": info["levels"]},
+        "label_build_s": ms_per_step / 1000.0,
+        "label_build_device_s": float(np.median(dev)) * 1e-6,
+        "label_build_plain_pf_s": min(plain_build_us) * 1e-6,
+        "label_build_ll_s": min(ll_build_us) * 1e-6,
+        "label_build_l3_s": min(l3_build_us) * 1e-6,
+        "contraction_device_s": contract_dev_us * 1e-6,
+        "contraction_wall_s": contract_wall,
+        "queries": q,
+        "batched_over_plain_lca": q["lca_batched_qps"] / q["lca_plain_qps"],
+        "latency_4k_ms": {"direct_p50": float(np.percentile(lat_direct, 50)),
+                          "direct_p99": float(np.percentile(lat_direct, 99)),
+                          "served_p50": float(np.percentile(lat_served, 50)),
+                          "served_p99": float(np.percentile(lat_served, 99)),
+                          "served_rate_qps": rate},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "label_items_kernel", "launches": launches,
+                     "kernel_us_per_build": kern_us, "algorithmic_bytes_per_build": b4},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["speedup_vs_cpu_label_build"] = cpu["label_build_s"] / (ms_per_step / 1000.0)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -257,6 +257,7 @@ h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int
     return set_err(ctx, H2H_ERR_ARG, "contract_order must be 0 (CPU order) or 1 (GPU order)");
   (void)hipSetDevice(ctx->device);
   ctx->contracted = ctx->tree_ready = ctx->labels_ready = false;
+  invalidate_label_plans(ctx);
   ctx->change_height = change_height;
   ctx->contract_order = contract_order;
   h2h_status st = build_partition_plans(ctx, change_height);
@@ -286,6 +287,18 @@ h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* us
   return run_label_build(ctx, label_kind, method, usec);
 }
 
+h2h_status h2h_build_labels_profiled(h2h_ctx* ctx, int label_kind, int method, double* kernel_us, int64_t* launches) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (!ctx->tree_ready) return set_err(ctx, H2H_ERR_STATE, "contract (or upload an index) first");
+  if (label_kind != H2H_LABEL_L3 && label_kind != H2H_LABEL_L4) return set_err(ctx, H2H_ERR_ARG, "label kind must be 3 or 4");
+  if (method != H2H_METHOD_PF && method != H2H_METHOD_LL && method != H2H_METHOD_BATCHED)
+    return set_err(ctx, H2H_ERR_ARG, "method must be PF (1), LL (2) or BATCHED (3)");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before rebuilding labels");
+  (void)hipSetDevice(ctx->device);
+  return profile_label_build(ctx, label_kind, method, kernel_us, launches);
+}
+
 h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, const int32_t* dis,
                              const int64_t* pos_hash, const int32_t* pos_id, const int32_t* pos_pos,
                              const int32_t* father, const int32_t* depth) {
@@ -295,6 +308,7 @@ h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, c
   if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the index");
   (void)hipSetDevice(ctx->device);
   ctx->tree_ready = ctx->labels_ready = ctx->contracted = false;
+  invalidate_label_plans(ctx);
   ctx->n = n;
   // bags from pos (self excluded), weights from the initial dis rows
   ctx->bag_off.assign(n + 1, 0);

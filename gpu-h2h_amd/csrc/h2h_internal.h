@@ -63,6 +63,27 @@ struct QueryWorkspace {
   int64_t cap = 0;
 };
 
+// Launch plan of one label-build method over the current tree: work items
+// grouped by level, the nodes whose rows start at INF, and (for the batched
+// method) the captured launch sequence per label kind.
+struct LabelPlan {
+  int method = -1;
+  std::vector<int32_t> lvl_off;    // [levels+1] item offsets
+  std::vector<int32_t> lvl_block;  // threads per block for each level
+  DevArray items, init_nodes, init_depth;
+  int64_t n_items = 0, n_init = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0] L3, [1] L4
+  void reset() {
+    for (auto& g : graph)
+      if (g) { (void)hipGraphExecDestroy(g); g = nullptr; }
+    method = -1;
+    lvl_off.clear();
+    lvl_block.clear();
+    n_items = n_init = 0;
+  }
+  ~LabelPlan() { reset(); }
+};
+
 struct Request;   // serving
 struct Server;
 
@@ -117,6 +138,7 @@ struct h2h_ctx {
   h2h::DevArray d_qx, d_qy, d_qout;   // staging for host-buffer queries
   size_t q_cap = 0;
   h2h::QueryWorkspace qws;
+  h2h::LabelPlan lplan[4];            // indexed by H2H_METHOD_*
   int lca_levels = 0;
   int num_cus = 256;
 
@@ -154,6 +176,8 @@ void merge_cpu_order(h2h_ctx* ctx, int thread_num);
 h2h_status run_contraction(h2h_ctx* ctx, double* usec);               // contract.hip
 h2h_status upload_index(h2h_ctx* ctx);                                 // labels.hip
 h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec);  // labels.hip
+void invalidate_label_plans(h2h_ctx* ctx);                             // labels.hip
+h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kernel_us, int64_t* launches);
 h2h_status build_lca(h2h_ctx* ctx);                                    // query.hip
 h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, const int32_t* dy,
                         int32_t* dout, int64_t n, hipStream_t s, QueryWorkspace* ws);  // query.hip

@@ -18,17 +18,22 @@
 // not change any entry.
 //
 // Kernels
-//   label_pf_kernel   one workgroup per node of a level (parallelFine = 1):
-//                     lanes own columns j (coalesced reads of ancestor rows),
-//                     the workgroup's waves split the bag entries and combine
-//                     through LDS.
-//   label_ll_*        one wavefront per (node, bag entry) of a level with an
-//                     atomic min into the node's row (parallelFine = 2, the
-//                     rule of kernel_functions.cu:717-744).
-//   label_dataflow    one persistent launch over the whole tree: workgroups
-//                     take nodes in BFS order from a counter and start a node
-//                     as soon as its father's row is published (no level
-//                     barriers), method H2H_METHOD_BATCHED.
+//   label_items_kernel  one workgroup per work item = (node, range of its bag
+//                       entries) of one level: lanes own columns j (coalesced
+//                       reads of the ancestor rows), waves split the entries and
+//                       combine through LDS; items sharing a node combine with
+//                       atomicMin.
+//   label_init_kernel   INF rows for nodes split over several items.
+// Methods (work-item plans over the same kernel)
+//   PF       one item per node (parallelFine = 1, kernel_functions.cu:803-842)
+//   LL       one item per (node, bag entry) (parallelFine = 2, the atomicMin
+//            rule of kernel_functions.cu:717-744)
+//   BATCHED  per level, one item per node where the level is wide and bags
+//            split into groups where it is narrow; the whole sweep is captured
+//            once in a hipGraph and replayed.
+// A workgroup that waits for another workgroup's row inside one launch needs
+// a device-scope release per row, which on this part writes back the L2; the
+// sweep therefore synchronises levels at kernel boundaries instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,57 +55,103 @@ struct LabelArgs {
   const int32_t* pos_w;
   const int64_t* pos_rowoff;
   int32_t* dis;
-  int32_t* anc;  // ancestor ids, same layout as dis (L4 only)
+  uint32_t* anc;  // row offset of the ancestor at each column, same layout as dis (L4 only)
   int l4;
 };
 
-// Computes columns [0, L) of node v's row with `nw` waves of the calling
-// workgroup (wave index wid).  comb: 64*R ints of LDS.
-__device__ inline void node_row(const LabelArgs& a, int v, int d, int wid, int nw, int lane,
-                                int32_t* comb) {
+// One work item = (node, a contiguous range of its bag entries).  The block's
+// waves split the range, each lane owns R columns of a 512-column pass and
+// keeps the running minimum in registers; waves combine through LDS.  An item
+// that covers the whole bag stores its row; items that share a node combine
+// with one atomicMin per column (the row was set to INF beforehand).
+// For L4 the column j > p_u gathers dis[a_j][p_u]; a_j (j < depth-1) is read
+// from the father's ancestor row, so no item waits for its own node's row.
+struct Item {
+  int32_t node;   // index into bfs
+  int32_t first;  // first bag entry, relative to pos_hash[v]
+  int32_t count;  // bag entries of this item
+  int32_t flags;  // ITEM_DIRECT: covers the whole bag; ITEM_LEAD: writes self entry / ancestor row
+};
+constexpr int ITEM_DIRECT = 1;
+constexpr int ITEM_LEAD = 2;
+
+// Loads of one bag entry are issued as a batch (addresses first, then all R
+// loads, then the mins): a load guarded by a lane-divergent branch makes the
+// compiler wait for it before the next one.
+template <bool L4>
+__device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, int64_t e1, int flags, int wid,
+                                int nw, int lane, int32_t* comb) {
   const int64_t rowv = a.dis_hash[v];
-  const int64_t e0 = a.pos_hash[v];
-  const int64_t e1 = a.pos_hash[v + 1] - 1;  // the self entry is last
   const int L = d - 1;
+  const int nk = (e1 > e0 + wid) ? (int)((e1 - e0 - wid + nw - 1) / nw) : 0;
   int64_t ancf = 0;
-  if (d > 1 && a.l4) ancf = a.dis_hash[a.father[v]];
-  if (a.l4) {
+  if (L4 && d > 1) ancf = a.dis_hash[a.father[v]];
+  if (L4 && (flags & ITEM_LEAD)) {
     for (int t = wid * 64 + lane; t < L; t += nw * 64) a.anc[rowv + t] = a.anc[ancf + t];
-    if (wid == 0 && lane == 0) a.anc[rowv + L] = v;
+    if (wid == 0 && lane == 0) a.anc[rowv + L] = (uint32_t)rowv;
   }
   for (int c0 = 0; c0 < L; c0 += 64 * R) {
     int32_t acc[R];
-    int64_t aoff[R];
+    uint32_t aoff[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       acc[r] = INF32;
-      aoff[r] = 0;
       const int j = c0 + r * 64 + lane;
-      if (a.l4 && j < L) aoff[r] = a.dis_hash[a.anc[ancf + j]];
+      aoff[r] = (L4 && j < L) ? a.anc[ancf + j] : 0u;
     }
-    for (int64_t e = e0 + wid; e < e1; e += nw) {
-      const int p = a.pos_pos[e];
-      const int w = a.pos_w[e];
-      const int64_t uo = a.pos_rowoff[e];
+    // this wave's entries are e0 + wid + k * nw, k < nk; their (p, w, row)
+    // sit one per lane and are broadcast with readlane, two entries at a time
+    for (int kb = 0; kb < nk; kb += 64) {
+      int mp = 0, mw = 0;
+      int64_t mu = 0;
+      if (kb + lane < nk) {
+        const int64_t e = e0 + wid + (int64_t)(kb + lane) * nw;
+        mp = a.pos_pos[e];
+        mw = a.pos_w[e];
+        mu = a.pos_rowoff[e];
+      }
+      const int mlo = (int)(uint32_t)mu, mhi = (int)(mu >> 32);
+      const int kn = min(64, nk - kb);
+      int i = 0;
+      for (; i + 1 < kn; i += 2) {
+        const int p0 = __builtin_amdgcn_readlane(mp, i), p1 = __builtin_amdgcn_readlane(mp, i + 1);
+        const int w0 = __builtin_amdgcn_readlane(mw, i), w1 = __builtin_amdgcn_readlane(mw, i + 1);
+        const int64_t u0 = ((int64_t)__builtin_amdgcn_readlane(mhi, i) << 32) | (uint32_t)__builtin_amdgcn_readlane(mlo, i);
+        const int64_t u1 = ((int64_t)__builtin_amdgcn_readlane(mhi, i + 1) << 32) | (uint32_t)__builtin_amdgcn_readlane(mlo, i + 1);
+        int32_t v0[R], v1[R];
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        const int j = c0 + r * 64 + lane;
-        if (j < L) {
-          if (j <= p) {
-            acc[r] = min(acc[r], w + a.dis[uo + j]);
-          } else if (a.l4) {
-            acc[r] = min(acc[r], w + a.dis[aoff[r] + p]);
-          }
+        for (int r = 0; r < R; r++) {
+          const int j = c0 + r * 64 + lane;
+          // L3 reads a clamped, always valid address and discards it past p
+          v0[r] = a.dis[L4 ? ((j <= p0) ? u0 + j : (int64_t)aoff[r] + p0) : u0 + min(j, p0)];
+          v1[r] = a.dis[L4 ? ((j <= p1) ? u1 + j : (int64_t)aoff[r] + p1) : u1 + min(j, p1)];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int j = c0 + r * 64 + lane;
+          const bool in = j < L;
+          acc[r] = (in && (L4 || j <= p0)) ? min(acc[r], w0 + v0[r]) : acc[r];
+          acc[r] = (in && (L4 || j <= p1)) ? min(acc[r], w1 + v1[r]) : acc[r];
+        }
+      }
+      if (i < kn) {
+        const int p0 = __builtin_amdgcn_readlane(mp, i);
+        const int w0 = __builtin_amdgcn_readlane(mw, i);
+        const int64_t u0 = ((int64_t)__builtin_amdgcn_readlane(mhi, i) << 32) | (uint32_t)__builtin_amdgcn_readlane(mlo, i);
+        int32_t v0[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int j = c0 + r * 64 + lane;
+          v0[r] = a.dis[L4 ? ((j <= p0) ? u0 + j : (int64_t)aoff[r] + p0) : u0 + min(j, p0)];
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int j = c0 + r * 64 + lane;
+          acc[r] = ((j < L) && (L4 || j <= p0)) ? min(acc[r], w0 + v0[r]) : acc[r];
         }
       }
     }
-    if (nw == 1) {
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        const int j = c0 + r * 64 + lane;
-        if (j < L) a.dis[rowv + j] = acc[r];
-      }
-    } else {
+    if (nw > 1) {
       for (int t = wid * 64 + lane; t < 64 * R; t += nw * 64) comb[t] = INF32;
       __syncthreads();
 #pragma unroll
@@ -111,130 +162,61 @@ __device__ inline void node_row(const LabelArgs& a, int v, int d, int wid, int n
       __syncthreads();
       for (int t = wid * 64 + lane; t < 64 * R; t += nw * 64) {
         const int j = c0 + t;
-        if (j < L) a.dis[rowv + j] = comb[t];
+        if (j < L) {
+          if (flags & ITEM_DIRECT) a.dis[rowv + j] = comb[t];
+          else if (comb[t] != INF32) atomicMin(&a.dis[rowv + j], comb[t]);
+        }
       }
       __syncthreads();
+    } else {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int j = c0 + r * 64 + lane;
+        if (j < L) {
+          if (flags & ITEM_DIRECT) a.dis[rowv + j] = acc[r];
+          else if (acc[r] != INF32) atomicMin(&a.dis[rowv + j], acc[r]);
+        }
+      }
     }
   }
-  if (wid == 0 && lane == 0) a.dis[rowv + L] = 0;
+  if ((flags & ITEM_LEAD) && wid == 0 && lane == 0) a.dis[rowv + L] = 0;
 }
 
-__global__ void __launch_bounds__(1024) label_pf_kernel(LabelArgs a, int lvl_begin, int d) {
+// Workgroups go to the 8 XCDs round-robin; block b takes item
+// (b % 8) * chunk + b / 8 so that each XCD works through a contiguous run of
+// the level's items (siblings, which read the same ancestor rows, share an L2).
+constexpr int XCDS = 8;
+
+template <bool L4>
+__global__ void __launch_bounds__(256) label_items_kernel(LabelArgs a, const Item* __restrict__ items, int begin,
+                                                        int count, int d) {
   __shared__ int32_t comb[64 * R];
-  const int v = a.bfs[lvl_begin + blockIdx.x];
-  node_row(a, v, d, threadIdx.x >> 6, blockDim.x >> 6, threadIdx.x & 63, comb);
+  const int chunk = (count + XCDS - 1) / XCDS;
+  const int idx = (int)(blockIdx.x % XCDS) * chunk + (int)(blockIdx.x / XCDS);
+  if (idx >= count) return;
+  const Item it = items[begin + idx];
+  const int v = a.bfs[it.node];
+  const int64_t e0 = a.pos_hash[v] + it.first;
+  item_row<L4>(a, v, d, e0, e0 + it.count, it.flags, threadIdx.x >> 6, blockDim.x >> 6, threadIdx.x & 63, comb);
 }
 
-// ---- LL: row initialisation as mallocH2HLabel_noHub leaves it, then one
-// wavefront per (node, bag entry) with atomic min (kernel_functions.cu:717-744).
-__global__ void __launch_bounds__(256) label_ll_init_kernel(LabelArgs a, int lvl_begin, int count, int d) {
+void launch_items(const LabelArgs& a, int count, dim3 block, hipStream_t s, const Item* items, int begin, int d) {
+  const dim3 grid((unsigned)(((count + XCDS - 1) / XCDS) * XCDS));
+  if (a.l4) hipLaunchKernelGGL(label_items_kernel<true>, grid, block, 0, s, a, items, begin, count, d);
+  else hipLaunchKernelGGL(label_items_kernel<false>, grid, block, 0, s, a, items, begin, count, d);
+}
+
+// Rows of nodes whose items combine with atomicMin start at INF (self 0).
+__global__ void __launch_bounds__(256) label_init_kernel(LabelArgs a, const int32_t* __restrict__ nodes,
+                                                       const int32_t* __restrict__ depth_of, int count) {
   const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wave >= count) return;
-  const int v = a.bfs[lvl_begin + wave];
+  const int v = a.bfs[nodes[wave]];
+  const int d = depth_of[wave];
   const int64_t rowv = a.dis_hash[v];
-  const int L = d - 1;
-  for (int j = lane; j < L; j += 64) a.dis[rowv + j] = INF32;
-  if (a.l4 && d > 1) {
-    const int64_t ancf = a.dis_hash[a.father[v]];
-    for (int j = lane; j < L; j += 64) a.anc[rowv + j] = a.anc[ancf + j];
-  }
-  // bag weights at the bag positions (positions are distinct)
-  for (int64_t e = a.pos_hash[v] + lane; e < a.pos_hash[v + 1] - 1; e += 64) a.dis[rowv + a.pos_pos[e]] = a.pos_w[e];
-  if (lane == 0) {
-    a.dis[rowv + L] = 0;
-    if (a.l4) a.anc[rowv + L] = v;
-  }
-}
-
-__global__ void __launch_bounds__(256) label_ll_kernel(LabelArgs a, const int32_t* ent_node,
-                                                      const int64_t* ent_pos, int64_t begin, int64_t count, int d) {
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= count) return;
-  const int v = ent_node[begin + wave];
-  const int64_t e = ent_pos[begin + wave];
-  const int p = a.pos_pos[e];
-  const int64_t rowv = a.dis_hash[v];
-  const int64_t uo = a.pos_rowoff[e];
-  const int t = __hip_atomic_load(&a.dis[rowv + p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (int j = lane; j < p; j += 64) atomicMin(&a.dis[rowv + j], t + a.dis[uo + j]);
-  if (a.l4) {
-    for (int j = p + 1 + lane; j < d - 1; j += 64) {
-      const int64_t ao = a.dis_hash[a.anc[rowv + j]];
-      atomicMin(&a.dis[rowv + j], t + a.dis[ao + p]);
-    }
-  }
-}
-
-// ---- dataflow: one persistent launch; a node starts once its father's row
-// is published.  Nodes are taken in BFS order, so every father was taken
-// earlier by a running workgroup and the waits end.
-struct FlowArgs {
-  LabelArgs a;
-  const int32_t* bfs_depth;   // depth of bfs[i]
-  const int32_t* bfs_index;   // [n] position of v in bfs
-  int32_t* done;              // [n] 1 once the row of bfs[i] is published
-  int32_t* next;              // work counter
-  int32_t* err;               // set if a wait exceeds its spin limit
-  int n;
-  int spin_limit;
-};
-
-// No lane-divergent branch may sit in this loop: with `if (threadIdx.x == 0)`
-// around the fetch and the publish, the compiler joins the two across the
-// back edge into a region only lane 0 enters, and the other lanes loop back to
-// the barrier on the stale item.  Wave 0 therefore fetches, waits and
-// publishes as a whole (every lane runs the same atomic / load / store).
-__global__ void __launch_bounds__(256) label_dataflow_kernel(FlowArgs f) {
-  __shared__ int32_t comb[64 * R];
-  __shared__ int32_t s_item;
-  __shared__ int32_t s_ok;
-  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = threadIdx.x & 63;
-  for (;;) {
-    if (wid == 0) {
-      // one increment per block: lane 0 adds 1, the other lanes add 0
-      int t = atomicAdd(f.next, lane == 0 ? 1 : 0);
-      t = __shfl(t, 0, 64);
-      s_item = t;
-    }
-    __syncthreads();
-    const int item = s_item;
-    if (item >= f.n) break;
-    const int v = f.a.bfs[item];
-    const int d = f.bfs_depth[item];
-    if (wid == 0) {
-      int ok = 1;
-      if (d > 1) {
-        const int fi = f.bfs_index[f.a.father[v]];
-        int spins = 0;
-        while (__hip_atomic_load(&f.done[fi], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-          if (__hip_atomic_load(f.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > f.spin_limit) {
-            ok = 0;
-            break;
-          }
-          __builtin_amdgcn_s_sleep(2);
-        }
-        if (!ok) __hip_atomic_store(f.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      s_ok = ok;
-    }
-    __syncthreads();
-    if (s_ok) node_row(f.a, v, d, wid, nw, lane, comb);
-    // every wave's row stores reach device scope before the row is published
-    __threadfence();
-    __syncthreads();
-    if (wid == 0) __hip_atomic_store(&f.done[item], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
-
-int pick_wpn(int64_t nodes, int64_t entries) {
-  const double avg = nodes ? (double)entries / nodes : 0.0;
-  int w = 1;
-  while (w < 16 && w * 8 < avg) w *= 2;
-  if (nodes >= 4096) w = std::min(w, 2);
-  if (nodes >= 16384) w = 1;
-  return w;
+  for (int j = lane; j < d - 1; j += 64) a.dis[rowv + j] = INF32;
+  if (lane == 0) a.dis[rowv + d - 1] = 0;
 }
 
 }  // namespace
@@ -244,6 +226,85 @@ int pick_wpn(int64_t nodes, int64_t entries) {
     h2h_status s_ = (x);         \
     if (s_ != H2H_OK) return s_; \
   } while (0)
+
+namespace {
+
+// Work-item plan of one method.
+//   PF       one item per node (its whole bag), 256 threads.
+//   LL       one item per (node, bag entry), 64 threads, atomic combine.
+//   BATCHED  per level: wide levels (>= 4 items per CU) one item per node;
+//            narrow levels split large bags into groups so the level still
+//            spreads over the chip; 256 threads.
+h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
+  P.reset();
+  const int levels = (int)ctx->level_off.size() - 1;
+  std::vector<Item> items;
+  std::vector<int32_t> init_nodes, init_depth;
+  P.lvl_off.assign(1, 0);
+  const int64_t target = (int64_t)ctx->num_cus * 4;
+  for (int l = 0; l < levels; l++) {
+    const int b0 = ctx->level_off[l], b1 = ctx->level_off[l + 1];
+    const int cnt = b1 - b0;
+    int64_t entries = 0;
+    for (int i = b0; i < b1; i++) {
+      const int v = ctx->bfs[i];
+      entries += ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1;
+    }
+    int64_t group = INT32_MAX;  // entries per item
+    int block = 256;
+    if (method == H2H_METHOD_LL) {
+      group = 1;
+      block = 64;
+    } else if (method == H2H_METHOD_BATCHED && cnt < target) {
+      group = std::max<int64_t>(4, (entries + target - 1) / target);
+    }
+    for (int i = b0; i < b1; i++) {
+      const int v = ctx->bfs[i];
+      const int32_t b = (int32_t)(ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1);
+      const int32_t nitems = b == 0 ? 1 : (int32_t)((b + group - 1) / group);
+      for (int32_t k = 0; k < nitems; k++) {
+        Item it;
+        it.node = i;
+        it.first = (int32_t)(k * std::min<int64_t>(group, b));
+        it.count = (int32_t)std::min<int64_t>(group, b - it.first);
+        if (b == 0) it.first = it.count = 0;
+        it.flags = (nitems == 1 ? ITEM_DIRECT : 0) | (k == 0 ? ITEM_LEAD : 0);
+        items.push_back(it);
+      }
+      if (nitems > 1) {
+        init_nodes.push_back(i);
+        init_depth.push_back(l + 1);
+      }
+    }
+    P.lvl_off.push_back((int32_t)items.size());
+    P.lvl_block.push_back(block);
+  }
+  H2H_TRY(upload(ctx, P.items, items.data(), items.size() * sizeof(Item)));
+  H2H_TRY(upload(ctx, P.init_nodes, init_nodes.data(), init_nodes.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.init_depth, init_depth.data(), init_depth.size() * sizeof(int32_t)));
+  P.n_items = (int64_t)items.size();
+  P.n_init = (int64_t)init_nodes.size();
+  P.method = method;
+  return H2H_OK;
+}
+
+void enqueue_plan(const LabelPlan& P, const LabelArgs& a, hipStream_t s) {
+  if (P.n_init > 0)
+    hipLaunchKernelGGL(label_init_kernel, dim3((unsigned)((P.n_init + 3) / 4)), dim3(256), 0, s, a,
+                       P.init_nodes.as<int32_t>(), P.init_depth.as<int32_t>(), (int)P.n_init);
+  const int levels = (int)P.lvl_off.size() - 1;
+  for (int l = 0; l < levels; l++) {
+    const int cnt = P.lvl_off[l + 1] - P.lvl_off[l];
+    if (cnt == 0) continue;
+    launch_items(a, cnt, dim3(P.lvl_block[l]), s, P.items.as<Item>(), P.lvl_off[l], l + 1);
+  }
+}
+
+}  // namespace
+
+void invalidate_label_plans(h2h_ctx* ctx) {
+  for (auto& P : ctx->lplan) P.reset();
+}
 
 h2h_status upload_index(h2h_ctx* ctx) {
   const int n = ctx->n;
@@ -266,10 +327,11 @@ h2h_status upload_index(h2h_ctx* ctx) {
 }
 
 h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
-  const int n = ctx->n;
   hipStream_t s = ctx->stream;
   const bool l4 = (kind == H2H_LABEL_L4);
-  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(int32_t)));
+  if (l4 && ctx->label_entries >= (int64_t)1 << 32)
+    return set_err(ctx, H2H_ERR_OOM, "L4 build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
+  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
   LabelArgs a;
   a.bfs = ctx->d_bfs.as<int32_t>();
   a.father = ctx->d_father.as<int32_t>();
@@ -279,88 +341,28 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   a.pos_w = ctx->d_pos_w.as<int32_t>();
   a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
   a.dis = ctx->d_dis.as<int32_t>();
-  a.anc = l4 ? ctx->d_anc.as<int32_t>() : nullptr;
+  a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
   a.l4 = l4 ? 1 : 0;
-  const int levels = (int)ctx->level_off.size() - 1;
-
-  // per-level bag entry counts (entries exclude the self entry)
-  std::vector<int64_t> lvl_entries(levels, 0);
-  for (int l = 0; l < levels; l++)
-    for (int i = ctx->level_off[l]; i < ctx->level_off[l + 1]; i++) {
-      int v = ctx->bfs[i];
-      lvl_entries[l] += ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1;
-    }
-
-  DevArray d_ent_node, d_ent_pos, d_flow_depth, d_flow_index, d_done, d_next, d_err;
-  std::vector<int64_t> ent_begin;
-  if (method == H2H_METHOD_LL) {
-    std::vector<int32_t> en;
-    std::vector<int64_t> ep;
-    ent_begin.push_back(0);
-    for (int l = 0; l < levels; l++) {
-      for (int i = ctx->level_off[l]; i < ctx->level_off[l + 1]; i++) {
-        int v = ctx->bfs[i];
-        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++) {
-          en.push_back(v);
-          ep.push_back(e);
-        }
-      }
-      ent_begin.push_back((int64_t)en.size());
-    }
-    H2H_TRY(upload(ctx, d_ent_node, en.data(), en.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, d_ent_pos, ep.data(), ep.size() * sizeof(int64_t)));
-  } else if (method == H2H_METHOD_BATCHED) {
-    std::vector<int32_t> bd(n), bi(n);
-    for (int l = 0; l < levels; l++)
-      for (int i = ctx->level_off[l]; i < ctx->level_off[l + 1]; i++) {
-        bd[i] = l + 1;
-        bi[ctx->bfs[i]] = i;
-      }
-    H2H_TRY(upload(ctx, d_flow_depth, bd.data(), n * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, d_flow_index, bi.data(), n * sizeof(int32_t)));
-    H2H_TRY(dev_alloc(ctx, d_done, n * sizeof(int32_t)));
-    H2H_TRY(dev_alloc(ctx, d_next, sizeof(int32_t)));
-    H2H_TRY(dev_alloc(ctx, d_err, sizeof(int32_t)));
+  LabelPlan& P = ctx->lplan[method];
+  if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
+  if (method == H2H_METHOD_BATCHED && !P.graph[l4]) {
+    // capture the whole level sweep once; replays skip per-launch host work
+    hipGraph_t g = nullptr;
+    H2H_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue_plan(P, a, s);
+    H2H_HIP(ctx, hipStreamEndCapture(s, &g));
+    hipError_t e = hipGraphInstantiate(&P.graph[l4], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_err(ctx, e, "hipGraphInstantiate");
   }
-
   hipEvent_t e0, e1;
   H2H_HIP(ctx, hipEventCreate(&e0));
   H2H_HIP(ctx, hipEventCreate(&e1));
-  if (method == H2H_METHOD_BATCHED) {
-    H2H_HIP(ctx, hipMemsetAsync(d_done.p, 0, n * sizeof(int32_t), s));
-    H2H_HIP(ctx, hipMemsetAsync(d_next.p, 0, sizeof(int32_t), s));
-    H2H_HIP(ctx, hipMemsetAsync(d_err.p, 0, sizeof(int32_t), s));
-  }
   H2H_HIP(ctx, hipEventRecord(e0, s));
-  if (method == H2H_METHOD_PF) {
-    for (int l = 0; l < levels; l++) {
-      const int cnt = ctx->level_off[l + 1] - ctx->level_off[l];
-      const int wpn = pick_wpn(cnt, lvl_entries[l]);
-      hipLaunchKernelGGL(label_pf_kernel, dim3(cnt), dim3(64 * wpn), 0, s, a, ctx->level_off[l], l + 1);
-    }
-  } else if (method == H2H_METHOD_LL) {
-    for (int l = 0; l < levels; l++) {
-      const int cnt = ctx->level_off[l + 1] - ctx->level_off[l];
-      hipLaunchKernelGGL(label_ll_init_kernel, dim3((cnt + 3) / 4), dim3(256), 0, s, a, ctx->level_off[l], cnt, l + 1);
-      const int64_t ecnt = ent_begin[l + 1] - ent_begin[l];
-      if (ecnt > 0)
-        hipLaunchKernelGGL(label_ll_kernel, dim3((unsigned)((ecnt + 3) / 4)), dim3(256), 0, s, a,
-                           d_ent_node.as<int32_t>(), d_ent_pos.as<int64_t>(), ent_begin[l], ecnt, l + 1);
-    }
+  if (method == H2H_METHOD_BATCHED) {
+    H2H_HIP(ctx, hipGraphLaunch(P.graph[l4], s));
   } else {
-    FlowArgs f;
-    f.a = a;
-    f.bfs_depth = d_flow_depth.as<int32_t>();
-    f.bfs_index = d_flow_index.as<int32_t>();
-    f.done = d_done.as<int32_t>();
-    f.next = d_next.as<int32_t>();
-    f.err = d_err.as<int32_t>();
-    f.n = n;
-    f.spin_limit = 1 << 20;  // about a second of polling: far beyond any father's row
-    int cus = 256;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, ctx->device) == hipSuccess) cus = prop.multiProcessorCount;
-    hipLaunchKernelGGL(label_dataflow_kernel, dim3(cus * 4), dim3(256), 0, s, f);
+    enqueue_plan(P, a, s);
   }
   H2H_HIP(ctx, hipGetLastError());
   H2H_HIP(ctx, hipEventRecord(e1, s));
@@ -369,13 +371,62 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   H2H_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (method == H2H_METHOD_BATCHED) {
-    int32_t herr = 0;
-    H2H_HIP(ctx, hipMemcpy(&herr, d_err.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-    if (herr) return set_err(ctx, H2H_ERR_HIP, "dataflow label build: a wait exceeded its spin limit");
-  }
   ctx->t_label_us = ms * 1000.0;
   if (usec) *usec = ctx->t_label_us;
+  ctx->label_kind = kind;
+  ctx->labels_ready = true;
+  return H2H_OK;
+}
+
+h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kernel_us, int64_t* launches) {
+  hipStream_t s = ctx->stream;
+  const bool l4 = (kind == H2H_LABEL_L4);
+  if (l4 && ctx->label_entries >= (int64_t)1 << 32)
+    return set_err(ctx, H2H_ERR_OOM, "L4 build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
+  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
+  LabelArgs a;
+  a.bfs = ctx->d_bfs.as<int32_t>();
+  a.father = ctx->d_father.as<int32_t>();
+  a.dis_hash = ctx->d_dis_hash.as<int64_t>();
+  a.pos_hash = ctx->d_pos_hash.as<int64_t>();
+  a.pos_pos = ctx->d_pos_pos.as<int32_t>();
+  a.pos_w = ctx->d_pos_w.as<int32_t>();
+  a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
+  a.dis = ctx->d_dis.as<int32_t>();
+  a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
+  a.l4 = l4 ? 1 : 0;
+  LabelPlan& P = ctx->lplan[method];
+  if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
+  const int levels = (int)P.lvl_off.size() - 1;
+  std::vector<hipEvent_t> ev(2 * levels + 2, nullptr);
+  for (auto& e : ev) H2H_HIP(ctx, hipEventCreate(&e));
+  int64_t nl = 0;
+  if (P.n_init > 0) {
+    H2H_HIP(ctx, hipEventRecord(ev[2 * levels], s));
+    hipLaunchKernelGGL(label_init_kernel, dim3((unsigned)((P.n_init + 3) / 4)), dim3(256), 0, s, a,
+                       P.init_nodes.as<int32_t>(), P.init_depth.as<int32_t>(), (int)P.n_init);
+    H2H_HIP(ctx, hipEventRecord(ev[2 * levels + 1], s));
+  }
+  for (int l = 0; l < levels; l++) {
+    const int cnt = P.lvl_off[l + 1] - P.lvl_off[l];
+    H2H_HIP(ctx, hipEventRecord(ev[2 * l], s));
+    if (cnt > 0) {
+      launch_items(a, cnt, dim3(P.lvl_block[l]), s, P.items.as<Item>(), P.lvl_off[l], l + 1);
+      nl++;
+    }
+    H2H_HIP(ctx, hipEventRecord(ev[2 * l + 1], s));
+  }
+  H2H_HIP(ctx, hipGetLastError());
+  H2H_HIP(ctx, hipStreamSynchronize(s));
+  double tot = 0;
+  for (int l = 0; l < levels; l++) {
+    float ms = 0;
+    H2H_HIP(ctx, hipEventElapsedTime(&ms, ev[2 * l], ev[2 * l + 1]));
+    tot += ms * 1000.0;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (kernel_us) *kernel_us = tot;
+  if (launches) *launches = nl;
   ctx->label_kind = kind;
   ctx->labels_ready = true;
   return H2H_OK;

@@ -75,6 +75,7 @@ def lib():
             "h2h_partition_rank_tree": [P, i32p, i32p, dp],
             "h2h_contract": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp],
             "h2h_build_labels": [P, ctypes.c_int, ctypes.c_int, dp],
+            "h2h_build_labels_profiled": [P, ctypes.c_int, ctypes.c_int, dp, i64p],
             "h2h_labels_upload": [P, ctypes.c_int32, i64p, i32p, i64p, i32p, i32p, i32p, i32p],
             "h2h_query": [P, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int64, dp],
             "h2h_query_device": [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64, P],
@@ -182,6 +183,13 @@ class H2HIndex:
         us = ctypes.c_double()
         self._chk(lib().h2h_build_labels(self.h, int(kind), int(method), ctypes.byref(us)))
         return us.value
+
+    def build_labels_profiled(self, kind=L4, method=BATCHED):
+        """(sum of label-kernel device microseconds, number of launches)."""
+        us = ctypes.c_double()
+        n = ctypes.c_int64()
+        self._chk(lib().h2h_build_labels_profiled(self.h, int(kind), int(method), ctypes.byref(us), ctypes.byref(n)))
+        return us.value, n.value
 
     def labels_upload(self, dis_hash, dis, pos_hash, pos_id, pos_pos, father, depth):
         arrs = [np.ascontiguousarray(dis_hash, np.int64), np.ascontiguousarray(dis, np.int32),

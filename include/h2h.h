@@ -120,6 +120,13 @@ h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int
  * resident.  usec: device time (hipEvent) of the build kernels. */
 h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* usec);
 
+/* The same build with an event pair around every level launch (no hipGraph):
+ * kernel_us is the sum of the label kernels' device durations and launches
+ * their count.  No counterpart in the reference: it exists to measure the
+ * build kernel against the HBM roofline. */
+h2h_status h2h_build_labels_profiled(h2h_ctx* ctx, int label_kind, int method, double* kernel_us,
+                                     int64_t* launches);
+
 /* Bring-your-own index: the arrays mallocH2HLabel_noHub produces
  * (H2HConsturction.cu:618-722): dis_hash[n+1], dis[dis_hash[n]] (bag weights at
  * bag positions, 0 at depth-1, anything elsewhere), pos_hash[n+1],
