@@ -57,6 +57,7 @@ struct LabelArgs {
   int32_t* dis;
   uint32_t* anc;  // row offset of the ancestor at each column, same layout as dis (L4 only)
   int l4;
+  int idx32;      // every dis byte offset fits 32 bits
 };
 
 // One work item = (node, a contiguous range of its bag entries).  The block's
@@ -77,8 +78,16 @@ constexpr int ITEM_LEAD = 2;
 
 // Loads of one bag entry are issued as a batch (addresses first, then all R
 // loads, then the mins): a load guarded by a lane-divergent branch makes the
-// compiler wait for it before the next one.
-template <bool L4>
+// compiler wait for it before the next one.  With IDX32 every dis index is a
+// 32-bit byte offset from the (scalar) base, so each load needs one address
+// register instead of two.
+template <bool IDX32>
+__device__ inline int32_t ld(const int32_t* base, uint64_t idx) {
+  if (IDX32) return *(const int32_t*)((const char*)base + (uint32_t)idx * 4u);
+  return base[idx];
+}
+
+template <bool L4, bool IDX32>
 __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, int64_t e1, int flags, int wid,
                                 int nw, int lane, int32_t* comb) {
   const int64_t rowv = a.dis_hash[v];
@@ -90,41 +99,50 @@ __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, in
     for (int t = wid * 64 + lane; t < L; t += nw * 64) a.anc[rowv + t] = a.anc[ancf + t];
     if (wid == 0 && lane == 0) a.anc[rowv + L] = (uint32_t)rowv;
   }
+  // this wave's entries are e0 + wid + k * nw, k < nk; their (p, w, row) sit
+  // one per lane (64 at a time) and are broadcast with readlane
+  int mp = 0, mw = 0;
+  uint64_t mu = 0;
+  auto load_meta = [&](int kb) {
+    mp = 0;
+    mw = 0;
+    mu = 0;
+    if (kb + lane < nk) {
+      const int64_t e = e0 + wid + (int64_t)(kb + lane) * nw;
+      mp = a.pos_pos[e];
+      mw = a.pos_w[e];
+      mu = (uint64_t)a.pos_rowoff[e];
+    }
+  };
+  load_meta(0);
   for (int c0 = 0; c0 < L; c0 += 64 * R) {
     int32_t acc[R];
-    uint32_t aoff[R];
+    uint64_t aoff[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       acc[r] = INF32;
       const int j = c0 + r * 64 + lane;
       aoff[r] = (L4 && j < L) ? a.anc[ancf + j] : 0u;
     }
-    // this wave's entries are e0 + wid + k * nw, k < nk; their (p, w, row)
-    // sit one per lane and are broadcast with readlane, two entries at a time
     for (int kb = 0; kb < nk; kb += 64) {
-      int mp = 0, mw = 0;
-      int64_t mu = 0;
-      if (kb + lane < nk) {
-        const int64_t e = e0 + wid + (int64_t)(kb + lane) * nw;
-        mp = a.pos_pos[e];
-        mw = a.pos_w[e];
-        mu = a.pos_rowoff[e];
-      }
+      if (kb > 0 || (c0 > 0 && nk > 64)) load_meta(kb);
       const int mlo = (int)(uint32_t)mu, mhi = (int)(mu >> 32);
       const int kn = min(64, nk - kb);
       int i = 0;
       for (; i + 1 < kn; i += 2) {
         const int p0 = __builtin_amdgcn_readlane(mp, i), p1 = __builtin_amdgcn_readlane(mp, i + 1);
         const int w0 = __builtin_amdgcn_readlane(mw, i), w1 = __builtin_amdgcn_readlane(mw, i + 1);
-        const int64_t u0 = ((int64_t)__builtin_amdgcn_readlane(mhi, i) << 32) | (uint32_t)__builtin_amdgcn_readlane(mlo, i);
-        const int64_t u1 = ((int64_t)__builtin_amdgcn_readlane(mhi, i + 1) << 32) | (uint32_t)__builtin_amdgcn_readlane(mlo, i + 1);
+        const uint64_t u0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane(mlo, i);
+        const uint64_t u1 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i + 1) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane(mlo, i + 1);
         int32_t v0[R], v1[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const int j = c0 + r * 64 + lane;
           // L3 reads a clamped, always valid address and discards it past p
-          v0[r] = a.dis[L4 ? ((j <= p0) ? u0 + j : (int64_t)aoff[r] + p0) : u0 + min(j, p0)];
-          v1[r] = a.dis[L4 ? ((j <= p1) ? u1 + j : (int64_t)aoff[r] + p1) : u1 + min(j, p1)];
+          v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : aoff[r] + p0) : u0 + min(j, p0));
+          v1[r] = ld<IDX32>(a.dis, L4 ? ((j <= p1) ? u1 + j : aoff[r] + p1) : u1 + min(j, p1));
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -137,12 +155,13 @@ __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, in
       if (i < kn) {
         const int p0 = __builtin_amdgcn_readlane(mp, i);
         const int w0 = __builtin_amdgcn_readlane(mw, i);
-        const int64_t u0 = ((int64_t)__builtin_amdgcn_readlane(mhi, i) << 32) | (uint32_t)__builtin_amdgcn_readlane(mlo, i);
+        const uint64_t u0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane(mlo, i);
         int32_t v0[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
           const int j = c0 + r * 64 + lane;
-          v0[r] = a.dis[L4 ? ((j <= p0) ? u0 + j : (int64_t)aoff[r] + p0) : u0 + min(j, p0)];
+          v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : aoff[r] + p0) : u0 + min(j, p0));
         }
 #pragma unroll
         for (int r = 0; r < R; r++) {
@@ -187,7 +206,7 @@ __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, in
 // the level's items (siblings, which read the same ancestor rows, share an L2).
 constexpr int XCDS = 8;
 
-template <bool L4>
+template <bool L4, bool IDX32>
 __global__ void __launch_bounds__(256) label_items_kernel(LabelArgs a, const Item* __restrict__ items, int begin,
                                                         int count, int d) {
   __shared__ int32_t comb[64 * R];
@@ -197,13 +216,19 @@ __global__ void __launch_bounds__(256) label_items_kernel(LabelArgs a, const Ite
   const Item it = items[begin + idx];
   const int v = a.bfs[it.node];
   const int64_t e0 = a.pos_hash[v] + it.first;
-  item_row<L4>(a, v, d, e0, e0 + it.count, it.flags, threadIdx.x >> 6, blockDim.x >> 6, threadIdx.x & 63, comb);
+  item_row<L4, IDX32>(a, v, d, e0, e0 + it.count, it.flags, threadIdx.x >> 6, blockDim.x >> 6, threadIdx.x & 63,
+                      comb);
 }
 
 void launch_items(const LabelArgs& a, int count, dim3 block, hipStream_t s, const Item* items, int begin, int d) {
   const dim3 grid((unsigned)(((count + XCDS - 1) / XCDS) * XCDS));
-  if (a.l4) hipLaunchKernelGGL(label_items_kernel<true>, grid, block, 0, s, a, items, begin, count, d);
-  else hipLaunchKernelGGL(label_items_kernel<false>, grid, block, 0, s, a, items, begin, count, d);
+  if (a.idx32) {
+    if (a.l4) hipLaunchKernelGGL((label_items_kernel<true, true>), grid, block, 0, s, a, items, begin, count, d);
+    else hipLaunchKernelGGL((label_items_kernel<false, true>), grid, block, 0, s, a, items, begin, count, d);
+  } else {
+    if (a.l4) hipLaunchKernelGGL((label_items_kernel<true, false>), grid, block, 0, s, a, items, begin, count, d);
+    else hipLaunchKernelGGL((label_items_kernel<false, false>), grid, block, 0, s, a, items, begin, count, d);
+  }
 }
 
 // Rows of nodes whose items combine with atomicMin start at INF (self 0).
@@ -343,6 +368,7 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   a.dis = ctx->d_dis.as<int32_t>();
   a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
   a.l4 = l4 ? 1 : 0;
+  a.idx32 = ctx->label_entries < ((int64_t)1 << 30) ? 1 : 0;
   LabelPlan& P = ctx->lplan[method];
   if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
   if (method == H2H_METHOD_BATCHED && !P.graph[l4]) {
@@ -395,6 +421,7 @@ h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kerne
   a.dis = ctx->d_dis.as<int32_t>();
   a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
   a.l4 = l4 ? 1 : 0;
+  a.idx32 = ctx->label_entries < ((int64_t)1 << 30) ? 1 : 0;
   LabelPlan& P = ctx->lplan[method];
   if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
   const int levels = (int)P.lvl_off.size() - 1;
