@@ -72,6 +72,9 @@ struct LabelPlan {
   std::vector<int32_t> lvl_block;  // threads per block for each level
   DevArray items, init_nodes, init_depth;
   int64_t n_items = 0, n_init = 0;
+  DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // persistent subtree tasks
+  int ntasks = 0;
+  int64_t task_nodes_n = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0] L3, [1] L4
   void reset() {
     for (auto& g : graph)
@@ -80,6 +83,8 @@ struct LabelPlan {
     lvl_off.clear();
     lvl_block.clear();
     n_items = n_init = 0;
+    ntasks = 0;
+    task_nodes_n = 0;
   }
   ~LabelPlan() { reset(); }
 };

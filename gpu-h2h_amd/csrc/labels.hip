@@ -87,6 +87,68 @@ __device__ inline int32_t ld(const int32_t* base, uint64_t idx) {
   return base[idx];
 }
 
+// Accumulates bag entries e0 + k * stride (k < nk) into the columns
+// [c0, c0 + 64R) of a row of length L + 1.  The entries' (p, w, row) sit one
+// per lane (64 at a time) and are broadcast with readlane, two entries per
+// round of loads.
+template <bool L4, bool IDX32>
+__device__ inline void accumulate(const LabelArgs& a, int64_t e0, int stride, int nk, int c0, int L, int lane,
+                                  const uint32_t* aoff, int32_t* acc) {
+  for (int kb = 0; kb < nk; kb += 64) {
+    int mp = 0, mw = 0;
+    uint64_t mu = 0;
+    if (kb + lane < nk) {
+      const int64_t e = e0 + (int64_t)(kb + lane) * stride;
+      mp = a.pos_pos[e];
+      mw = a.pos_w[e];
+      mu = (uint64_t)a.pos_rowoff[e];
+    }
+    const int mlo = (int)(uint32_t)mu, mhi = (int)(mu >> 32);
+    const int kn = min(64, nk - kb);
+    int i = 0;
+    for (; i + 1 < kn; i += 2) {
+      const int p0 = __builtin_amdgcn_readlane(mp, i), p1 = __builtin_amdgcn_readlane(mp, i + 1);
+      const int w0 = __builtin_amdgcn_readlane(mw, i), w1 = __builtin_amdgcn_readlane(mw, i + 1);
+      const uint64_t u0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane(mlo, i);
+      const uint64_t u1 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i + 1) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane(mlo, i + 1);
+      int32_t v0[R], v1[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int j = c0 + r * 64 + lane;
+        // L3 reads a clamped, always valid address and discards it past p
+        v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : (uint64_t)aoff[r] + p0) : u0 + min(j, p0));
+        v1[r] = ld<IDX32>(a.dis, L4 ? ((j <= p1) ? u1 + j : (uint64_t)aoff[r] + p1) : u1 + min(j, p1));
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int j = c0 + r * 64 + lane;
+        const bool in = j < L;
+        acc[r] = (in && (L4 || j <= p0)) ? min(acc[r], w0 + v0[r]) : acc[r];
+        acc[r] = (in && (L4 || j <= p1)) ? min(acc[r], w1 + v1[r]) : acc[r];
+      }
+    }
+    if (i < kn) {
+      const int p0 = __builtin_amdgcn_readlane(mp, i);
+      const int w0 = __builtin_amdgcn_readlane(mw, i);
+      const uint64_t u0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i) << 32) |
+                          (uint32_t)__builtin_amdgcn_readlane(mlo, i);
+      int32_t v0[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int j = c0 + r * 64 + lane;
+        v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : (uint64_t)aoff[r] + p0) : u0 + min(j, p0));
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int j = c0 + r * 64 + lane;
+        acc[r] = ((j < L) && (L4 || j <= p0)) ? min(acc[r], w0 + v0[r]) : acc[r];
+      }
+    }
+  }
+}
+
 template <bool L4, bool IDX32>
 __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, int64_t e1, int flags, int wid,
                                 int nw, int lane, int32_t* comb) {
@@ -99,77 +161,16 @@ __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, in
     for (int t = wid * 64 + lane; t < L; t += nw * 64) a.anc[rowv + t] = a.anc[ancf + t];
     if (wid == 0 && lane == 0) a.anc[rowv + L] = (uint32_t)rowv;
   }
-  // this wave's entries are e0 + wid + k * nw, k < nk; their (p, w, row) sit
-  // one per lane (64 at a time) and are broadcast with readlane
-  int mp = 0, mw = 0;
-  uint64_t mu = 0;
-  auto load_meta = [&](int kb) {
-    mp = 0;
-    mw = 0;
-    mu = 0;
-    if (kb + lane < nk) {
-      const int64_t e = e0 + wid + (int64_t)(kb + lane) * nw;
-      mp = a.pos_pos[e];
-      mw = a.pos_w[e];
-      mu = (uint64_t)a.pos_rowoff[e];
-    }
-  };
-  load_meta(0);
   for (int c0 = 0; c0 < L; c0 += 64 * R) {
     int32_t acc[R];
-    uint64_t aoff[R];
+    uint32_t aoff[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
       acc[r] = INF32;
       const int j = c0 + r * 64 + lane;
       aoff[r] = (L4 && j < L) ? a.anc[ancf + j] : 0u;
     }
-    for (int kb = 0; kb < nk; kb += 64) {
-      if (kb > 0 || (c0 > 0 && nk > 64)) load_meta(kb);
-      const int mlo = (int)(uint32_t)mu, mhi = (int)(mu >> 32);
-      const int kn = min(64, nk - kb);
-      int i = 0;
-      for (; i + 1 < kn; i += 2) {
-        const int p0 = __builtin_amdgcn_readlane(mp, i), p1 = __builtin_amdgcn_readlane(mp, i + 1);
-        const int w0 = __builtin_amdgcn_readlane(mw, i), w1 = __builtin_amdgcn_readlane(mw, i + 1);
-        const uint64_t u0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i) << 32) |
-                            (uint32_t)__builtin_amdgcn_readlane(mlo, i);
-        const uint64_t u1 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i + 1) << 32) |
-                            (uint32_t)__builtin_amdgcn_readlane(mlo, i + 1);
-        int32_t v0[R], v1[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const int j = c0 + r * 64 + lane;
-          // L3 reads a clamped, always valid address and discards it past p
-          v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : aoff[r] + p0) : u0 + min(j, p0));
-          v1[r] = ld<IDX32>(a.dis, L4 ? ((j <= p1) ? u1 + j : aoff[r] + p1) : u1 + min(j, p1));
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const int j = c0 + r * 64 + lane;
-          const bool in = j < L;
-          acc[r] = (in && (L4 || j <= p0)) ? min(acc[r], w0 + v0[r]) : acc[r];
-          acc[r] = (in && (L4 || j <= p1)) ? min(acc[r], w1 + v1[r]) : acc[r];
-        }
-      }
-      if (i < kn) {
-        const int p0 = __builtin_amdgcn_readlane(mp, i);
-        const int w0 = __builtin_amdgcn_readlane(mw, i);
-        const uint64_t u0 = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(mhi, i) << 32) |
-                            (uint32_t)__builtin_amdgcn_readlane(mlo, i);
-        int32_t v0[R];
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const int j = c0 + r * 64 + lane;
-          v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : aoff[r] + p0) : u0 + min(j, p0));
-        }
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-          const int j = c0 + r * 64 + lane;
-          acc[r] = ((j < L) && (L4 || j <= p0)) ? min(acc[r], w0 + v0[r]) : acc[r];
-        }
-      }
-    }
+    accumulate<L4, IDX32>(a, e0 + wid, nw, nk, c0, L, lane, aoff, acc);
     if (nw > 1) {
       for (int t = wid * 64 + lane; t < 64 * R; t += nw * 64) comb[t] = INF32;
       __syncthreads();
@@ -199,6 +200,39 @@ __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, in
     }
   }
   if ((flags & ITEM_LEAD) && wid == 0 && lane == 0) a.dis[rowv + L] = 0;
+}
+
+// One wavefront computes the columns [c0, c0 + 64R) of node v's row from its
+// whole bag and stores them; the unit with c0 = 0 also writes the self entry.
+template <bool L4, bool IDX32>
+__device__ inline void unit_row(const LabelArgs& a, int v, int d, int c0, int lane) {
+  const int64_t rowv = a.dis_hash[v];
+  const int L = d - 1;
+  const int64_t e0 = a.pos_hash[v];
+  const int nk = (int)(a.pos_hash[v + 1] - 1 - e0);
+  int64_t ancf = 0;
+  if (L4 && d > 1) ancf = a.dis_hash[a.father[v]];
+  if (L4) {
+    const int jend = min(c0 + 64 * R, L);
+    for (int j = c0 + lane; j < jend; j += 64) a.anc[rowv + j] = a.anc[ancf + j];
+    if (c0 == 0 && lane == 0) a.anc[rowv + L] = (uint32_t)rowv;
+  }
+  if (c0 == 0 && lane == 0) a.dis[rowv + L] = 0;
+  if (c0 >= L) return;
+  int32_t acc[R];
+  uint32_t aoff[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    acc[r] = INF32;
+    const int j = c0 + r * 64 + lane;
+    aoff[r] = (L4 && j < L) ? a.anc[ancf + j] : 0u;
+  }
+  accumulate<L4, IDX32>(a, e0, 1, nk, c0, L, lane, aoff, acc);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = c0 + r * 64 + lane;
+    if (j < L) a.dis[rowv + j] = acc[r];
+  }
 }
 
 // Workgroups go to the 8 XCDs round-robin; block b takes item
@@ -244,6 +278,48 @@ __global__ void __launch_bounds__(256) label_init_kernel(LabelArgs a, const int3
   if (lane == 0) a.dis[rowv + d - 1] = 0;
 }
 
+// Subtree tasks: a maximal subtree of at most a few hundred nodes depends only
+// on itself and on ancestors finished by earlier launches, so one workgroup
+// walks it level by level with workgroup barriers (its waves share one CU,
+// hence one L1; no device-scope fence is needed between its levels).  The
+// waves take (node, column pass) units of a level round-robin.
+struct TaskArgs {
+  LabelArgs a;
+  const int32_t* lvl_base;   // [ntasks+1] first level of each task
+  const int32_t* lvl_off;    // [levels+1] node offsets of every task level
+  const int32_t* lvl_depth;  // [levels] depth of the level's nodes
+  const int32_t* nodes;      // vertex ids, task by task, level by level
+  int ntasks;
+};
+
+template <bool L4, bool IDX32>
+__global__ void __launch_bounds__(1024) label_subtree_kernel(TaskArgs t) {
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = threadIdx.x & 63;
+  const int chunk = (t.ntasks + XCDS - 1) / XCDS;
+  const int task = (int)(blockIdx.x % XCDS) * chunk + (int)(blockIdx.x / XCDS);
+  if (task >= t.ntasks) return;
+  const int l1 = t.lvl_base[task + 1];
+  for (int li = t.lvl_base[task]; li < l1; li++) {
+    const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
+    const int d = t.lvl_depth[li];
+    const int passes = d > 1 ? (d - 1 + 64 * R - 1) / (64 * R) : 1;
+    const int units = (n1 - n0) * passes;
+    for (int u = wid; u < units; u += nw) unit_row<L4, IDX32>(t.a, t.nodes[n0 + u / passes], d, (u % passes) * 64 * R, lane);
+    __syncthreads();
+  }
+}
+
+void launch_subtrees(const TaskArgs& t, hipStream_t s) {
+  const dim3 grid((unsigned)(((t.ntasks + XCDS - 1) / XCDS) * XCDS)), block(1024);
+  if (t.a.idx32) {
+    if (t.a.l4) hipLaunchKernelGGL((label_subtree_kernel<true, true>), grid, block, 0, s, t);
+    else hipLaunchKernelGGL((label_subtree_kernel<false, true>), grid, block, 0, s, t);
+  } else {
+    if (t.a.l4) hipLaunchKernelGGL((label_subtree_kernel<true, false>), grid, block, 0, s, t);
+    else hipLaunchKernelGGL((label_subtree_kernel<false, false>), grid, block, 0, s, t);
+  }
+}
+
 }  // namespace
 
 #define H2H_TRY(x)               \
@@ -263,6 +339,55 @@ namespace {
 h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
   P.reset();
   const int levels = (int)ctx->level_off.size() - 1;
+  const int n = ctx->n;
+  // BATCHED: maximal subtrees of at most smax nodes become persistent tasks
+  std::vector<int32_t> task_of(n, -1);
+  if (method == H2H_METHOD_BATCHED) {
+    const int64_t smax = std::max<int64_t>(64, n / (2LL * ctx->num_cus));
+    std::vector<int64_t> sub(n, 1);
+    for (int i = n - 1; i > 0; i--) {
+      const int v = ctx->bfs[i];
+      sub[ctx->father[v]] += sub[v];
+    }
+    int ntasks = 0;
+    for (int i = 0; i < n; i++) {
+      const int v = ctx->bfs[i];
+      const int f = ctx->father[v];
+      if (f >= 0 && task_of[f] >= 0) task_of[v] = task_of[f];
+      else if (sub[v] <= smax && f >= 0) task_of[v] = ntasks++;
+    }
+    std::vector<int32_t> order;  // task nodes, task by task, BFS order inside
+    std::vector<int32_t> cnt(ntasks + 1, 0);
+    for (int i = 0; i < n; i++)
+      if (task_of[ctx->bfs[i]] >= 0) cnt[task_of[ctx->bfs[i]] + 1]++;
+    for (int t = 0; t < ntasks; t++) cnt[t + 1] += cnt[t];
+    order.assign(cnt[ntasks], 0);
+    {
+      std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
+      for (int i = 0; i < n; i++) {
+        const int v = ctx->bfs[i];
+        if (task_of[v] >= 0) order[fill[task_of[v]]++] = v;
+      }
+    }
+    // levels of every task: start offsets into `order` and their depth
+    std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth;
+    for (int t = 0; t < ntasks; t++) {
+      for (int k = cnt[t]; k < cnt[t + 1]; k++) {
+        if (k == cnt[t] || ctx->depth[order[k]] != ctx->depth[order[k - 1]]) {
+          starts.push_back(k);
+          lvl_depth.push_back(ctx->depth[order[k]]);
+        }
+      }
+      lvl_base.push_back((int32_t)lvl_depth.size());
+    }
+    starts.push_back((int32_t)order.size());
+    H2H_TRY(upload(ctx, P.task_lvl_base, lvl_base.data(), lvl_base.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_lvl_off, starts.data(), starts.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_lvl_depth, lvl_depth.data(), lvl_depth.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_nodes, order.data(), order.size() * sizeof(int32_t)));
+    P.ntasks = ntasks;
+    P.task_nodes_n = (int64_t)order.size();
+  }
   std::vector<Item> items;
   std::vector<int32_t> init_nodes, init_depth;
   P.lvl_off.assign(1, 0);
@@ -271,8 +396,11 @@ h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
     const int b0 = ctx->level_off[l], b1 = ctx->level_off[l + 1];
     const int cnt = b1 - b0;
     int64_t entries = 0;
+    int cnt_top = 0;
     for (int i = b0; i < b1; i++) {
       const int v = ctx->bfs[i];
+      if (task_of[v] >= 0) continue;
+      cnt_top++;
       entries += ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1;
     }
     int64_t group = INT32_MAX;  // entries per item
@@ -280,11 +408,13 @@ h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
     if (method == H2H_METHOD_LL) {
       group = 1;
       block = 64;
-    } else if (method == H2H_METHOD_BATCHED && cnt < target) {
+    } else if (method == H2H_METHOD_BATCHED && cnt_top < target) {
       group = std::max<int64_t>(4, (entries + target - 1) / target);
     }
+    (void)cnt;
     for (int i = b0; i < b1; i++) {
       const int v = ctx->bfs[i];
+      if (task_of[v] >= 0) continue;
       const int32_t b = (int32_t)(ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1);
       const int32_t nitems = b == 0 ? 1 : (int32_t)((b + group - 1) / group);
       for (int32_t k = 0; k < nitems; k++) {
@@ -322,6 +452,16 @@ void enqueue_plan(const LabelPlan& P, const LabelArgs& a, hipStream_t s) {
     const int cnt = P.lvl_off[l + 1] - P.lvl_off[l];
     if (cnt == 0) continue;
     launch_items(a, cnt, dim3(P.lvl_block[l]), s, P.items.as<Item>(), P.lvl_off[l], l + 1);
+  }
+  if (P.ntasks > 0) {
+    TaskArgs t;
+    t.a = a;
+    t.lvl_base = P.task_lvl_base.as<int32_t>();
+    t.lvl_off = P.task_lvl_off.as<int32_t>();
+    t.lvl_depth = P.task_lvl_depth.as<int32_t>();
+    t.nodes = P.task_nodes.as<int32_t>();
+    t.ntasks = P.ntasks;
+    launch_subtrees(t, s);
   }
 }
 
@@ -443,6 +583,22 @@ h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kerne
     }
     H2H_HIP(ctx, hipEventRecord(ev[2 * l + 1], s));
   }
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  if (P.ntasks > 0) {
+    H2H_HIP(ctx, hipEventCreate(&t0));
+    H2H_HIP(ctx, hipEventCreate(&t1));
+    TaskArgs t;
+    t.a = a;
+    t.lvl_base = P.task_lvl_base.as<int32_t>();
+    t.lvl_off = P.task_lvl_off.as<int32_t>();
+    t.lvl_depth = P.task_lvl_depth.as<int32_t>();
+    t.nodes = P.task_nodes.as<int32_t>();
+    t.ntasks = P.ntasks;
+    H2H_HIP(ctx, hipEventRecord(t0, s));
+    launch_subtrees(t, s);
+    H2H_HIP(ctx, hipEventRecord(t1, s));
+    nl++;
+  }
   H2H_HIP(ctx, hipGetLastError());
   H2H_HIP(ctx, hipStreamSynchronize(s));
   double tot = 0;
@@ -450,6 +606,13 @@ h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kerne
     float ms = 0;
     H2H_HIP(ctx, hipEventElapsedTime(&ms, ev[2 * l], ev[2 * l + 1]));
     tot += ms * 1000.0;
+  }
+  if (t0) {
+    float ms = 0;
+    H2H_HIP(ctx, hipEventElapsedTime(&ms, t0, t1));
+    tot += ms * 1000.0;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (kernel_us) *kernel_us = tot;
