@@ -75,6 +75,19 @@ struct LabelPlan {
   DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // persistent subtree tasks
   int ntasks = 0;
   int64_t task_nodes_n = 0;
+  // BATCHED top part: the tree above the subtree tasks as vertical chains
+  DevArray chains, tnode, touter, tinner, inner_w, inner_k, inner_i, tpath, cunits;
+  struct Round {  // one chain depth: unit ranges of the four chain kernels
+    int k1b, k1n, k2b, k2n, k3b, k3n, k4b, k4n, lmax, imax;
+  };
+  std::vector<Round> rounds;
+  int kcl_b = 0, kcl_n = 0;  // units of the closure launch (every chain)
+  int64_t nt = 0;           // nodes in chains
+  int64_t tbo = 0;          // offset of the transposed chain rows inside dis
+  int64_t tro = 0;          // offset of the t_i(m) triangles (scratch) inside dis
+  int64_t pro = 0;          // offset of the closure triangles (scratch) inside dis
+  int64_t dis_entries = 0;  // dis entries the plan needs (labels, transposed rows, scratch)
+  const void* graph_dis[2] = {nullptr, nullptr};  // dis buffer each graph was captured with
   hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0] L3, [1] L4
   void reset() {
     for (auto& g : graph)
@@ -85,6 +98,10 @@ struct LabelPlan {
     n_items = n_init = 0;
     ntasks = 0;
     task_nodes_n = 0;
+    rounds.clear();
+    nt = tbo = tro = pro = dis_entries = 0;
+    kcl_b = kcl_n = 0;
+    graph_dis[0] = graph_dis[1] = nullptr;
   }
   ~LabelPlan() { reset(); }
 };

@@ -7,8 +7,8 @@
 //     p_u = depth(u)-1 with the shortcut weight w(v,u) and dis_hash[u], then
 //     v itself last.
 //
-// A node's row only depends on its ancestors' rows, so the tree is built
-// top-down level by level.  For L3 (as shipped) and L4 (exact) the row is
+// A node's row only depends on its ancestors' rows.  For L3 (as shipped) and
+// L4 (exact) the row is
 //   dis[v][j] = min over u in X(v) of  w(v,u) + (p_u >= j ? dis[u][j]
 //                                                   : L4 ? dis[a_j][p_u] : inf)
 // where a_j is v's ancestor at depth j+1.  This is the fixpoint the
@@ -17,27 +17,53 @@
 // update order and the use of w(v,u) instead of the running dis[v][p_u] do
 // not change any entry.
 //
-// Kernels
+// Level kernels (PF and LL, and the reference-shaped sweep)
 //   label_items_kernel  one workgroup per work item = (node, range of its bag
 //                       entries) of one level: lanes own columns j (coalesced
 //                       reads of the ancestor rows), waves split the entries and
 //                       combine through LDS; items sharing a node combine with
 //                       atomicMin.
 //   label_init_kernel   INF rows for nodes split over several items.
-// Methods (work-item plans over the same kernel)
 //   PF       one item per node (parallelFine = 1, kernel_functions.cu:803-842)
 //   LL       one item per (node, bag entry) (parallelFine = 2, the atomicMin
 //            rule of kernel_functions.cu:717-744)
-//   BATCHED  per level, one item per node where the level is wide and bags
-//            split into groups where it is narrow; the whole sweep is captured
-//            once in a hipGraph and replayed.
+//
+// BATCHED (one hipGraph, replayed)
+//   Maximal subtrees of at most n / (2 * CUs) nodes are persistent tasks of
+//   label_subtree_kernel (one workgroup walks its subtree level by level).
+//   The tree above them is cut into vertical chains of at most 128 nodes (a
+//   chain is a run of nodes each with a single child above the tasks: the
+//   separator vertices of one partition cell).  Chains of equal chain depth
+//   are independent, and a chain c_0..c_{L-1} with head depth h0 splits into
+//     outer columns j < h0-1 (ancestors above the chain): with B_i(j) the
+//       minimum over the bag entries outside the chain,
+//         dis[c_i][j] = min(B_i(j), min over c_k in X(c_i) of w + dis[c_k][j]),
+//       a recurrence down the chain along each column on its own;
+//     inner columns h0-1+m (the chain's own nodes), a triangle:
+//         T[i][m] = min(t_i(m), min over c_k in X(c_i) of
+//                        w + (k >= m ? T[k][m] : L4 ? T[m][k] : inf))
+//       with t_i(m) = min over entries u outside the chain of w + dis[c_m][p_u]
+//       (L4 only).
+//   Four launches per chain depth instead of one per tree level:
+//     chain_bound_kernel  B for every (chain node, 512-column pass), parallel;
+//     chain_fold_kernel   one workgroup per (chain, 128 columns) runs the
+//                         column recurrence in LDS;
+//     chain_cross_kernel  t_i(m), lanes over m (L4);
+//     chain_tri_kernel    one workgroup per chain runs the triangle in LDS.
+//   The L4 gathers dis[a_j][p] of chain and task rows read a transposed copy
+//   of the chain rows (tb[p][t] = dis[chain node t][p], written by the fold
+//   and triangle kernels) so that the 64 lanes of a gather hit consecutive
+//   words.
+//
 // A workgroup that waits for another workgroup's row inside one launch needs
-// a device-scope release per row, which on this part writes back the L2; the
-// sweep therefore synchronises levels at kernel boundaries instead.
+// a device-scope release per row, which on this part writes back the L2;
+// dependent steps are therefore separated by kernel boundaries.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
+#include <cstring>
 
 #include "h2h_internal.h"
 
@@ -58,6 +84,8 @@ struct LabelArgs {
   uint32_t* anc;  // row offset of the ancestor at each column, same layout as dis (L4 only)
   int l4;
   int idx32;      // every dis byte offset fits 32 bits
+  int64_t tbo;    // transposed chain rows: dis[tbo + p * nt + t] = dis[chain node t][p]
+  int64_t nt;     // chain nodes
 };
 
 // One work item = (node, a contiguous range of its bag entries).  The block's
@@ -87,11 +115,19 @@ __device__ inline int32_t ld(const int32_t* base, uint64_t idx) {
   return base[idx];
 }
 
+// Index of the L4 gather dis[a_j][p]: from the row offset of a_j, or (TBG)
+// from the chain index t of a_j in the transposed chain rows.
+template <bool TBG>
+__device__ inline uint64_t gather_index(const LabelArgs& a, uint32_t g, int p) {
+  return TBG ? (uint64_t)a.tbo + (uint64_t)(uint32_t)p * (uint64_t)a.nt + g : (uint64_t)g + (uint32_t)p;
+}
+
 // Accumulates bag entries e0 + k * stride (k < nk) into the columns
-// [c0, c0 + 64R) of a row of length L + 1.  The entries' (p, w, row) sit one
-// per lane (64 at a time) and are broadcast with readlane, two entries per
-// round of loads.
-template <bool L4, bool IDX32>
+// [c0, c0 + 64R) of a row of length L + 1 (columns j < L).  The entries'
+// (p, w, row) sit one per lane (64 at a time) and are broadcast with readlane,
+// two entries per round of loads.  aoff[r] locates a_j of the lane's column r
+// (see gather_index).
+template <bool L4, bool IDX32, bool TBG = false>
 __device__ inline void accumulate(const LabelArgs& a, int64_t e0, int stride, int nk, int c0, int L, int lane,
                                   const uint32_t* aoff, int32_t* acc) {
   for (int kb = 0; kb < nk; kb += 64) {
@@ -118,8 +154,8 @@ __device__ inline void accumulate(const LabelArgs& a, int64_t e0, int stride, in
       for (int r = 0; r < R; r++) {
         const int j = c0 + r * 64 + lane;
         // L3 reads a clamped, always valid address and discards it past p
-        v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : (uint64_t)aoff[r] + p0) : u0 + min(j, p0));
-        v1[r] = ld<IDX32>(a.dis, L4 ? ((j <= p1) ? u1 + j : (uint64_t)aoff[r] + p1) : u1 + min(j, p1));
+        v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : gather_index<TBG>(a, aoff[r], p0)) : u0 + min(j, p0));
+        v1[r] = ld<IDX32>(a.dis, L4 ? ((j <= p1) ? u1 + j : gather_index<TBG>(a, aoff[r], p1)) : u1 + min(j, p1));
       }
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -138,7 +174,7 @@ __device__ inline void accumulate(const LabelArgs& a, int64_t e0, int stride, in
 #pragma unroll
       for (int r = 0; r < R; r++) {
         const int j = c0 + r * 64 + lane;
-        v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : (uint64_t)aoff[r] + p0) : u0 + min(j, p0));
+        v0[r] = ld<IDX32>(a.dis, L4 ? ((j <= p0) ? u0 + j : gather_index<TBG>(a, aoff[r], p0)) : u0 + min(j, p0));
       }
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -320,6 +356,397 @@ void launch_subtrees(const TaskArgs& t, hipStream_t s) {
   }
 }
 
+
+// ---- chains (BATCHED top part) ----
+
+struct Chain {
+  int32_t first;  // chain index t of the head; a chain's nodes have consecutive indices
+  int32_t len;    // nodes (at most 128)
+  int32_t h0;     // depth of the head: the outer columns are [0, h0 - 1)
+  int32_t path;   // offset in tpath of the chain indices of the head's h0 - 1 ancestors
+  int32_t tri;    // offset (after tro) of the chain's triangle scratch, len (len + 1) / 2 entries
+  int32_t pad[3];
+};
+
+struct ChainArgs {
+  LabelArgs a;
+  const Chain* chains;
+  const int32_t* tnode;    // [nt] vertex of chain index t
+  const int32_t* touter;   // [nt] bag entries outside the node's chain (a prefix of its pos row)
+  const int32_t* tinner;   // [nt + 1] offsets of the bag entries inside the chain (chain-contiguous)
+  const int32_t* inner_w;  // their weights
+  const uint8_t* inner_k;  // index in the chain of the bag member
+  const uint8_t* inner_i;  // index in the chain of the node whose bag holds the entry
+  const int32_t* tpath;
+  const int4* units;       // per-kernel work units
+  int64_t tro;             // t_i(m) triangles (scratch inside dis)
+  int64_t pro;             // closure triangles P (scratch inside dis)
+  int begin, count;        // units of this launch
+};
+
+// blockIdx -> work index so that each XCD takes a contiguous run (see above)
+__device__ inline int xcd_index(int nblocks) {
+  const int chunk = (nblocks + XCDS - 1) / XCDS;
+  return (int)(blockIdx.x % XCDS) * chunk + (int)(blockIdx.x / XCDS);
+}
+
+inline dim3 xcd_grid(int nblocks) { return dim3((unsigned)(((nblocks + XCDS - 1) / XCDS) * XCDS)); }
+
+// B_i(j) for one (chain node, 512-column pass) per wavefront: the bag entries
+// outside the chain only.  Also writes the node's ancestor row (L4).
+template <bool L4, bool IDX32>
+__global__ void __launch_bounds__(256) chain_bound_kernel(ChainArgs c) {
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u = xcd_index((c.count + 3) >> 2) * 4 + wid;
+  if (u >= c.count) return;  // whole wavefronts; no barriers below
+  const LabelArgs& a = c.a;
+  const int4 un = c.units[c.begin + u];
+  const Chain ch = c.chains[un.x];
+  const int i = un.y, c0 = un.z;
+  const int t = ch.first + i;
+  const int v = c.tnode[t];
+  const int64_t rowv = a.dis_hash[v];
+  const int L = ch.h0 - 1;
+  if (L4) {
+    const int f = a.father[c.tnode[ch.first]];
+    const int64_t ancf = f >= 0 ? a.dis_hash[f] : 0;
+    const int jend = min(c0 + 64 * R, L);
+    for (int j = c0 + lane; j < jend; j += 64) a.anc[rowv + j] = a.anc[ancf + j];
+    if (c0 == 0)
+      for (int m = lane; m <= i; m += 64) a.anc[rowv + L + m] = (uint32_t)a.dis_hash[c.tnode[ch.first + m]];
+  }
+  if (c0 >= L) return;
+  int32_t acc[R];
+  uint32_t g[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    acc[r] = INF32;
+    const int j = c0 + r * 64 + lane;
+    g[r] = (L4 && j < L) ? (uint32_t)c.tpath[ch.path + j] : 0u;
+  }
+  accumulate<L4, IDX32, true>(a, a.pos_hash[v], 1, c.touter[t], c0, L, lane, g, acc);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = c0 + r * 64 + lane;
+    if (j < L) a.dis[rowv + j] = acc[r];
+  }
+}
+
+// Row offsets of the chain's nodes, one per lane (node q in lane q of
+// r0 and lane q - 64 of r1), broadcast with readlane: no scalar load chain
+// per row.  Built with every lane active; the broadcast reads r0 and r1
+// themselves (never a value computed under a partial exec mask, whose
+// inactive lanes hold stale data) and picks one with a uniform condition.
+struct ChainRows {
+  int64_t r0, r1;
+  __device__ ChainRows(const ChainArgs& c, const Chain& ch, int lane) {
+    r0 = c.a.dis_hash[c.tnode[ch.first + min(lane, ch.len - 1)]];
+    r1 = c.a.dis_hash[c.tnode[ch.first + min(lane + 64, ch.len - 1)]];
+  }
+  __device__ int64_t operator()(int i) const {
+    const int l = i & 63;
+    const uint32_t lo0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)r0, l);
+    const uint32_t hi0 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)r0 >> 32), l);
+    const uint32_t lo1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)r1, l);
+    const uint32_t hi1 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)r1 >> 32), l);
+    return i < 64 ? (int64_t)(((uint64_t)hi0 << 32) | lo0) : (int64_t)(((uint64_t)hi1 << 32) | lo1);
+  }
+};
+
+// Outer columns of a chain: x_i(j) = min over m <= i of P[i][m] + B_m(j),
+// with B the bounds chain_bound_kernel left in the rows and P the closure of
+// the chain's inner entries (chain_fw_kernel).  One workgroup per (chain,
+// 64 columns); the wavefronts take rows round-robin, lanes own columns.
+// LDS: the B tile [len][65] and the P triangle.
+template <bool IDX32>
+__global__ void __launch_bounds__(256) chain_apply_kernel(ChainArgs c) {
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int u = xcd_index(c.count);
+  if (u >= c.count) return;  // whole workgroup
+  const LabelArgs& a = c.a;
+  const int4 un = c.units[c.begin + u];
+  const Chain ch = c.chains[un.x];
+  const int L = ch.h0 - 1, n = ch.len;
+  const int tot = n * (n + 1) / 2;
+  int32_t* S = lds;           // S[m * 65 + lane] = B_m(j)
+  int32_t* P = lds + n * 65;  // P[i (i + 1) / 2 + m]
+  const int j = un.y + lane;
+  const int jc = min(j, L - 1);  // clamped: always a valid address
+  const ChainRows row(c, ch, lane);
+  for (int i0 = wid * 8; i0 < n; i0 += 32) {
+    int32_t x[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) x[q] = ld<IDX32>(a.dis, (uint64_t)row(min(i0 + q, n - 1)) + jc);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (i0 + q < n) S[(i0 + q) * 65 + lane] = x[q];
+  }
+  {
+    const uint64_t pb = (uint64_t)c.pro + ch.tri;
+    for (int q0 = 0; q0 < tot; q0 += 8 * 256) {
+      int32_t x[8];
+#pragma unroll
+      for (int r = 0; r < 8; r++) x[r] = ld<IDX32>(a.dis, pb + min(q0 + r * 256 + tid, tot - 1));
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+        if (q0 + r * 256 + tid < tot) P[q0 + r * 256 + tid] = x[r];
+    }
+  }
+  __syncthreads();
+  for (int i = wid; i < n; i += 4) {
+    const int pi = i * (i + 1) / 2;
+    int32_t x = INF32;
+    int m = 0;
+    for (; m + 8 <= i + 1; m += 8) {
+      int32_t y[8], pv[8];
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        pv[r] = P[pi + m + r];
+        y[r] = S[(m + r) * 65 + lane];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; r++) x = min(x, __builtin_elementwise_add_sat(pv[r], y[r]));
+    }
+    for (; m <= i; m++) x = min(x, __builtin_elementwise_add_sat(P[pi + m], S[m * 65 + lane]));
+    const int64_t ri = row(i);
+    if (j < L) {
+      a.dis[ri + j] = x;
+      a.dis[(uint64_t)a.tbo + (uint64_t)j * (uint64_t)a.nt + ch.first + i] = x;
+    }
+  }
+}
+
+// t_i(m) = min over the entries of c_i outside the chain of w + dis[c_m][p]
+// (L4), lanes over m, read from the transposed rows; written with the
+// diagonal 0 into row i of the chain's triangle scratch.
+template <bool IDX32>
+__global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u = xcd_index((c.count + 3) >> 2) * 4 + wid;
+  if (u >= c.count) return;
+  const LabelArgs& a = c.a;
+  const int4 un = c.units[c.begin + u];
+  const Chain ch = c.chains[un.x];
+  const int i = un.y, n = ch.len;
+  const int t = ch.first + i;
+  const int v = c.tnode[t];
+  const int nk = c.touter[t];
+  const int64_t e0 = a.pos_hash[v];
+  const int m0 = min(lane, n - 1), m1 = min(lane + 64, n - 1);
+  const bool two = i > 64;
+  int32_t x0 = INF32, x1 = INF32;
+  for (int kb = 0; kb < nk; kb += 64) {
+    int mp = 0, mw = 0;
+    if (kb + lane < nk) {
+      mp = a.pos_pos[e0 + kb + lane];
+      mw = a.pos_w[e0 + kb + lane];
+    }
+    const int kn = min(64, nk - kb);
+    int q = 0;
+    for (; q + 3 < kn; q += 4) {
+      int32_t y[4], z[4], w[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int p = __builtin_amdgcn_readlane(mp, q + r);
+        w[r] = __builtin_amdgcn_readlane(mw, q + r);
+        const uint64_t rb = (uint64_t)a.tbo + (uint64_t)(uint32_t)p * (uint64_t)a.nt + ch.first;
+        y[r] = ld<IDX32>(a.dis, rb + m0);
+        z[r] = two ? ld<IDX32>(a.dis, rb + m1) : INF32;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        x0 = min(x0, w[r] + y[r]);
+        if (two) x1 = min(x1, w[r] + z[r]);
+      }
+    }
+    for (; q < kn; q++) {
+      const int p = __builtin_amdgcn_readlane(mp, q);
+      const int w = __builtin_amdgcn_readlane(mw, q);
+      const uint64_t rb = (uint64_t)a.tbo + (uint64_t)(uint32_t)p * (uint64_t)a.nt + ch.first;
+      x0 = min(x0, w + ld<IDX32>(a.dis, rb + m0));
+      if (two) x1 = min(x1, w + ld<IDX32>(a.dis, rb + m1));
+    }
+  }
+  const uint64_t trow = (uint64_t)c.tro + ch.tri + (uint64_t)i * (i + 1) / 2;
+  if (lane <= i) a.dis[trow + lane] = lane == i ? 0 : x0;
+  if (lane + 64 <= i) a.dis[trow + lane + 64] = lane + 64 == i ? 0 : x1;
+}
+
+// Floyd-Warshall over one chain's nodes in a workgroup (256 threads, chains
+// of at most FW_N nodes): thread (column m, row block g) keeps the 64 cells
+// T[g * 64 + r][m] in registers; step k publishes row k (and, when the
+// matrix is not symmetric, column k) through LDS and every thread relaxes its
+// cells through k.  Cells past the chain stay INF (saturating adds).
+//   FW_CLOSURE  P = closure of the inner entries as a DAG (paths that go up
+//               the chain), lower triangle to the P scratch;
+//   FW_L3       the same closure is the L3 triangle: written to the rows;
+//   FW_L4       the symmetric graph of inner entries and t_i(m) edges: its
+//               all-pairs distances are the L4 triangle, written to the rows.
+// LDS: the matrix [FW_N][FW_LD] for set-up and output, row and column buffers.
+constexpr int FW_N = 128;
+constexpr int FW_RPT = 64;
+constexpr int FW_LD = FW_N + 1;  // odd stride: column accesses spread over the banks
+enum { FW_CLOSURE = 0, FW_L3 = 1, FW_L4 = 2 };
+size_t fw_lds() { return (size_t)(FW_N * FW_LD + 4 * FW_N) * 4; }
+
+// row i of a triangle stored row by row: the largest i with i (i + 1) / 2 <= q
+__device__ inline int tri_row(int q) {
+  int i = (int)((sqrtf(8.0f * (float)q + 1.0f) - 1.0f) * 0.5f);
+  while (i * (i + 1) / 2 > q) i--;
+  while ((i + 1) * (i + 2) / 2 <= q) i++;
+  return i;
+}
+
+template <int MODE, bool IDX32>
+__global__ void __launch_bounds__(256) chain_fw_kernel(ChainArgs c) {
+  constexpr bool SYM = MODE == FW_L4;
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, m = tid & (FW_N - 1), g = tid >> 7, lane = tid & 63;
+  const int u = xcd_index(c.count);
+  if (u >= c.count) return;  // whole workgroup
+  const LabelArgs& a = c.a;
+  const Chain ch = c.chains[c.units[c.begin + u].x];
+  const int n = ch.len, h = ch.h0 - 1;
+  const int tot = n * (n + 1) / 2;
+  int32_t* M = lds;
+  int32_t* Rb = lds + FW_N * FW_LD;  // [2][FW_N] row k
+  int32_t* Cb = Rb + 2 * FW_N;       // [2][FW_N] column k
+  for (int q = tid; q < FW_N * FW_LD; q += 256) M[q] = INF32;
+  __syncthreads();
+  if (SYM) {
+    // t_i(m) edges and the diagonal from the cross scratch
+    const uint64_t tb = (uint64_t)c.tro + ch.tri;
+    for (int q = tid; q < tot; q += 256) {
+      const int i = tri_row(q), mm = q - i * (i + 1) / 2;
+      const int32_t x = ld<IDX32>(a.dis, tb + q);
+      M[i * FW_LD + mm] = x;
+      M[mm * FW_LD + i] = x;
+    }
+  }
+  for (int i = tid; i < FW_N; i += 256) M[i * FW_LD + i] = 0;
+  __syncthreads();
+  {
+    const int ib = c.tinner[ch.first];
+    const int icnt = c.tinner[ch.first + n] - ib;
+    for (int q = tid; q < icnt; q += 256) {
+      const int i = c.inner_i[ib + q], k = c.inner_k[ib + q];
+      const int32_t w = c.inner_w[ib + q];
+      M[i * FW_LD + k] = min(M[i * FW_LD + k], w);
+      if (SYM) M[k * FW_LD + i] = min(M[k * FW_LD + i], w);
+    }
+  }
+  __syncthreads();
+  int32_t t[FW_RPT];
+#pragma unroll
+  for (int r = 0; r < FW_RPT; r++) t[r] = M[(g * FW_RPT + r) * FW_LD + m];
+  for (int k = 0; k < n; k++) {
+    const int b = (k & 1) * FW_N;
+    if (g == (k >> 6)) {
+      int32_t v = t[0];
+#pragma unroll
+      for (int r = 1; r < FW_RPT; r++) v = (r == (k & 63)) ? t[r] : v;
+      Rb[b + m] = v;
+    }
+    if (!SYM && m == k) {
+#pragma unroll
+      for (int r = 0; r < FW_RPT; r++) Cb[b + g * FW_RPT + r] = t[r];
+    }
+    __syncthreads();
+    const int32_t rkm = Rb[b + m];
+    const int32_t* col = SYM ? Rb + b + g * FW_RPT : Cb + b + g * FW_RPT;  // T[i][k] for this thread's rows
+#pragma unroll
+    for (int r = 0; r < FW_RPT; r++) t[r] = min(t[r], __builtin_elementwise_add_sat(col[r], rkm));
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < FW_RPT; r++) M[(g * FW_RPT + r) * FW_LD + m] = t[r];
+  __syncthreads();
+  if (MODE == FW_CLOSURE) {
+    const uint64_t pb = (uint64_t)c.pro + ch.tri;
+    for (int q = tid; q < tot; q += 256) {
+      const int i = tri_row(q);
+      a.dis[pb + q] = M[i * FW_LD + (q - i * (i + 1) / 2)];
+    }
+    return;
+  }
+  const ChainRows row(c, ch, lane);
+  for (int i = 0; i < n; i++) {
+    const int64_t ri = row(i) + h;
+    for (int mm = tid; mm <= i; mm += 256) a.dis[ri + mm] = M[i * FW_LD + mm];
+  }
+  for (int mm = 0; mm < n; mm++) {
+    const uint64_t tb = (uint64_t)a.tbo + (uint64_t)(h + mm) * (uint64_t)a.nt + ch.first;
+    for (int i = mm + tid; i < n; i += 256) a.dis[tb + i] = M[i * FW_LD + mm];
+  }
+}
+
+template <typename K>
+void set_lds_limit(K kernel) {
+  (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+}
+
+size_t apply_lds(int lmax) { return (size_t)(lmax * 65 + lmax * (lmax + 1) / 2) * 4; }
+
+template <int MODE>
+void launch_fw(const ChainArgs& c, int nunits, hipStream_t s) {
+  if (c.a.idx32) hipLaunchKernelGGL((chain_fw_kernel<MODE, true>), xcd_grid(nunits), dim3(256), fw_lds(), s, c);
+  else hipLaunchKernelGGL((chain_fw_kernel<MODE, false>), xcd_grid(nunits), dim3(256), fw_lds(), s, c);
+}
+
+// The closure P of every chain (one launch, ahead of the rounds).
+void launch_chain_closure(const ChainArgs& base, int begin, int count, hipStream_t s) {
+  ChainArgs c = base;
+  c.begin = begin;
+  c.count = count;
+  launch_fw<FW_CLOSURE>(c, count, s);
+}
+
+// Step `step` (0 bound, 1 apply, 2 cross, 3 triangle) of one chain depth.
+void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipStream_t s, int step) {
+  ChainArgs c = base;
+  const bool l4 = c.a.l4 != 0, i32 = c.a.idx32 != 0;
+  if (step == 0 && rd.k1n > 0) {
+    c.begin = rd.k1b;
+    c.count = rd.k1n;
+    const dim3 grid = xcd_grid((rd.k1n + 3) / 4);
+    if (i32) {
+      if (l4) hipLaunchKernelGGL((chain_bound_kernel<true, true>), grid, dim3(256), 0, s, c);
+      else hipLaunchKernelGGL((chain_bound_kernel<false, true>), grid, dim3(256), 0, s, c);
+    } else {
+      if (l4) hipLaunchKernelGGL((chain_bound_kernel<true, false>), grid, dim3(256), 0, s, c);
+      else hipLaunchKernelGGL((chain_bound_kernel<false, false>), grid, dim3(256), 0, s, c);
+    }
+  } else if (step == 1 && rd.k2n > 0) {
+    c.begin = rd.k2b;
+    c.count = rd.k2n;
+    const size_t lds = apply_lds(rd.lmax);
+    if (i32) hipLaunchKernelGGL((chain_apply_kernel<true>), xcd_grid(rd.k2n), dim3(256), lds, s, c);
+    else hipLaunchKernelGGL((chain_apply_kernel<false>), xcd_grid(rd.k2n), dim3(256), lds, s, c);
+  } else if (step == 2 && rd.k3n > 0 && l4) {
+    c.begin = rd.k3b;
+    c.count = rd.k3n;
+    const dim3 grid = xcd_grid((rd.k3n + 3) / 4);
+    if (i32) hipLaunchKernelGGL((chain_cross_kernel<true>), grid, dim3(256), 0, s, c);
+    else hipLaunchKernelGGL((chain_cross_kernel<false>), grid, dim3(256), 0, s, c);
+  } else if (step == 3 && rd.k4n > 0) {
+    c.begin = rd.k4b;
+    c.count = rd.k4n;
+    if (l4) launch_fw<FW_L4>(c, rd.k4n, s);
+    else launch_fw<FW_L3>(c, rd.k4n, s);
+  }
+}
+
+void set_chain_lds_limits() {
+  set_lds_limit(chain_apply_kernel<true>);
+  set_lds_limit(chain_apply_kernel<false>);
+  set_lds_limit(chain_fw_kernel<FW_CLOSURE, true>);
+  set_lds_limit(chain_fw_kernel<FW_CLOSURE, false>);
+  set_lds_limit(chain_fw_kernel<FW_L3, true>);
+  set_lds_limit(chain_fw_kernel<FW_L3, false>);
+  set_lds_limit(chain_fw_kernel<FW_L4, true>);
+  set_lds_limit(chain_fw_kernel<FW_L4, false>);
+}
 }  // namespace
 
 #define H2H_TRY(x)               \
@@ -330,20 +757,67 @@ void launch_subtrees(const TaskArgs& t, hipStream_t s) {
 
 namespace {
 
-// Work-item plan of one method.
-//   PF       one item per node (its whole bag), 256 threads.
+// Work-item plans.
+//   PF       one item per node (its whole bag), 256 threads, one launch per level.
 //   LL       one item per (node, bag entry), 64 threads, atomic combine.
-//   BATCHED  per level: wide levels (>= 4 items per CU) one item per node;
-//            narrow levels split large bags into groups so the level still
-//            spreads over the chip; 256 threads.
-h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
-  P.reset();
+//   BATCHED  subtree tasks below, chains above (see the top of this file).
+constexpr int CHAIN_MAX = 128;          // nodes per chain: the LDS tiles are sized for it
+constexpr int CHAIN_INNER_MAX = 16384;  // bag entries inside one chain
+
+// Tuning knobs of the batched plan (environment, read when a plan is built):
+// H2H_TASK_NODES caps the nodes of a subtree task (default n / (2 * CUs), at
+// least 64), H2H_CHAIN_NODES the nodes of a chain (1..128, default 128).
+int64_t env_int(const char* name, int64_t dflt, int64_t lo, int64_t hi) {
+  const char* v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  return std::min(hi, std::max(lo, (int64_t)std::atoll(v)));
+}
+
+h2h_status plan_items(h2h_ctx* ctx, int method, LabelPlan& P) {
   const int levels = (int)ctx->level_off.size() - 1;
+  std::vector<Item> items;
+  std::vector<int32_t> init_nodes, init_depth;
+  P.lvl_off.assign(1, 0);
+  for (int l = 0; l < levels; l++) {
+    const int b0 = ctx->level_off[l], b1 = ctx->level_off[l + 1];
+    const int64_t group = method == H2H_METHOD_LL ? 1 : INT32_MAX;  // entries per item
+    for (int i = b0; i < b1; i++) {
+      const int v = ctx->bfs[i];
+      const int32_t b = (int32_t)(ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1);
+      const int32_t nitems = b == 0 ? 1 : (int32_t)((b + group - 1) / group);
+      for (int32_t k = 0; k < nitems; k++) {
+        Item it;
+        it.node = i;
+        it.first = (int32_t)(k * std::min<int64_t>(group, b));
+        it.count = (int32_t)std::min<int64_t>(group, b - it.first);
+        if (b == 0) it.first = it.count = 0;
+        it.flags = (nitems == 1 ? ITEM_DIRECT : 0) | (k == 0 ? ITEM_LEAD : 0);
+        items.push_back(it);
+      }
+      if (nitems > 1) {
+        init_nodes.push_back(i);
+        init_depth.push_back(l + 1);
+      }
+    }
+    P.lvl_off.push_back((int32_t)items.size());
+    P.lvl_block.push_back(method == H2H_METHOD_LL ? 64 : 256);
+  }
+  H2H_TRY(upload(ctx, P.items, items.data(), items.size() * sizeof(Item)));
+  H2H_TRY(upload(ctx, P.init_nodes, init_nodes.data(), init_nodes.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.init_depth, init_depth.data(), init_depth.size() * sizeof(int32_t)));
+  P.n_items = (int64_t)items.size();
+  P.n_init = (int64_t)init_nodes.size();
+  P.dis_entries = ctx->label_entries;
+  return H2H_OK;
+}
+
+h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   const int n = ctx->n;
-  // BATCHED: maximal subtrees of at most smax nodes become persistent tasks
+  // maximal subtrees of at most smax nodes become persistent tasks
   std::vector<int32_t> task_of(n, -1);
-  if (method == H2H_METHOD_BATCHED) {
-    const int64_t smax = std::max<int64_t>(64, n / (2LL * ctx->num_cus));
+  const int64_t smax = env_int("H2H_TASK_NODES", std::max<int64_t>(64, n / (2LL * ctx->num_cus)), 1, INT32_MAX);
+  const int chain_max = (int)env_int("H2H_CHAIN_NODES", CHAIN_MAX, 1, CHAIN_MAX);
+  {
     std::vector<int64_t> sub(n, 1);
     for (int i = n - 1; i > 0; i--) {
       const int v = ctx->bfs[i];
@@ -388,70 +862,209 @@ h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
     P.ntasks = ntasks;
     P.task_nodes_n = (int64_t)order.size();
   }
-  std::vector<Item> items;
-  std::vector<int32_t> init_nodes, init_depth;
-  P.lvl_off.assign(1, 0);
-  const int64_t target = (int64_t)ctx->num_cus * 4;
-  for (int l = 0; l < levels; l++) {
-    const int b0 = ctx->level_off[l], b1 = ctx->level_off[l + 1];
-    const int cnt = b1 - b0;
-    int64_t entries = 0;
-    int cnt_top = 0;
-    for (int i = b0; i < b1; i++) {
-      const int v = ctx->bfs[i];
-      if (task_of[v] >= 0) continue;
-      cnt_top++;
-      entries += ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1;
-    }
-    int64_t group = INT32_MAX;  // entries per item
-    int block = 256;
-    if (method == H2H_METHOD_LL) {
-      group = 1;
-      block = 64;
-    } else if (method == H2H_METHOD_BATCHED && cnt_top < target) {
-      group = std::max<int64_t>(4, (entries + target - 1) / target);
-    }
-    (void)cnt;
-    for (int i = b0; i < b1; i++) {
-      const int v = ctx->bfs[i];
-      if (task_of[v] >= 0) continue;
-      const int32_t b = (int32_t)(ctx->pos_hash[v + 1] - ctx->pos_hash[v] - 1);
-      const int32_t nitems = b == 0 ? 1 : (int32_t)((b + group - 1) / group);
-      for (int32_t k = 0; k < nitems; k++) {
-        Item it;
-        it.node = i;
-        it.first = (int32_t)(k * std::min<int64_t>(group, b));
-        it.count = (int32_t)std::min<int64_t>(group, b - it.first);
-        if (b == 0) it.first = it.count = 0;
-        it.flags = (nitems == 1 ? ITEM_DIRECT : 0) | (k == 0 ? ITEM_LEAD : 0);
-        items.push_back(it);
-      }
-      if (nitems > 1) {
-        init_nodes.push_back(i);
-        init_depth.push_back(l + 1);
+
+  // chains over the other nodes, built in BFS order (fathers first): a node
+  // extends its father's chain when it is the father's only child outside
+  // the tasks and the chain has room
+  std::vector<int32_t> tchildren(n, 0);
+  for (int v = 0; v < n; v++)
+    if (task_of[v] < 0 && ctx->father[v] >= 0) tchildren[ctx->father[v]]++;
+  struct HostChain {
+    std::vector<int32_t> nodes;
+    int cd;          // chain depth
+    int64_t inner;   // bag entries inside the chain
+  };
+  std::vector<HostChain> hc;
+  std::vector<int32_t> chain_of(n, -1);
+  for (int b = 0; b < n; b++) {
+    const int v = ctx->bfs[b];
+    if (task_of[v] >= 0) continue;
+    const int f = ctx->father[v];
+    int c = -1;
+    if (f >= 0 && tchildren[f] == 1) {
+      HostChain& pc = hc[chain_of[f]];
+      const int h = ctx->depth[pc.nodes[0]] - 1;
+      int64_t cnt = 0;
+      for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++)
+        if (ctx->pos_pos[e] >= h) cnt++;
+      if ((int)pc.nodes.size() < chain_max && pc.inner + cnt <= CHAIN_INNER_MAX) {
+        c = chain_of[f];
+        pc.inner += cnt;
       }
     }
-    P.lvl_off.push_back((int32_t)items.size());
-    P.lvl_block.push_back(block);
+    if (c < 0) {
+      c = (int)hc.size();
+      hc.push_back({{}, f < 0 ? 0 : hc[chain_of[f]].cd + 1, 0});
+    }
+    chain_of[v] = c;
+    hc[c].nodes.push_back(v);
   }
-  H2H_TRY(upload(ctx, P.items, items.data(), items.size() * sizeof(Item)));
-  H2H_TRY(upload(ctx, P.init_nodes, init_nodes.data(), init_nodes.size() * sizeof(int32_t)));
-  H2H_TRY(upload(ctx, P.init_depth, init_depth.data(), init_depth.size() * sizeof(int32_t)));
-  P.n_items = (int64_t)items.size();
-  P.n_init = (int64_t)init_nodes.size();
+  std::vector<int> corder(hc.size());
+  for (size_t c = 0; c < hc.size(); c++) corder[c] = (int)c;
+  std::stable_sort(corder.begin(), corder.end(), [&](int x, int y) { return hc[x].cd < hc[y].cd; });
+  std::vector<int32_t> cpos(hc.size()), tidx(n, -1), tnode;
+  for (size_t ci = 0; ci < corder.size(); ci++) {
+    cpos[corder[ci]] = (int32_t)ci;
+    for (int v : hc[corder[ci]].nodes) {
+      tidx[v] = (int32_t)tnode.size();
+      tnode.push_back(v);
+    }
+  }
+  const int64_t nt = (int64_t)tnode.size();
+  std::vector<Chain> ch(corder.size());
+  std::vector<int32_t> tpath, touter(nt), tinner(nt + 1, 0), inner_w;
+  std::vector<uint8_t> inner_k, inner_i;
+  int64_t tri = 0;
+  int maxdepth = 0;
+  for (size_t ci = 0; ci < corder.size(); ci++) {
+    const auto& nodes = hc[corder[ci]].nodes;
+    Chain& x = ch[ci];
+    std::memset(&x, 0, sizeof(x));
+    x.first = tidx[nodes[0]];
+    x.len = (int32_t)nodes.size();
+    x.h0 = ctx->depth[nodes[0]];
+    x.tri = (int32_t)tri;
+    tri += (int64_t)x.len * (x.len + 1) / 2;
+    x.path = (int32_t)tpath.size();
+    const int f = ctx->father[nodes[0]];
+    if (f >= 0) {
+      const Chain& pc = ch[cpos[chain_of[f]]];
+      std::vector<int32_t> up(tpath.begin() + pc.path, tpath.begin() + pc.path + pc.h0 - 1);
+      for (int k = pc.first; k <= tidx[f]; k++) up.push_back(k);
+      tpath.insert(tpath.end(), up.begin(), up.end());
+    }
+    if ((int64_t)tpath.size() - x.path != x.h0 - 1)
+      return set_err(ctx, H2H_ERR_INPUT, "chain plan: ancestor path does not match the head depth");
+    const int h = x.h0 - 1;
+    for (int i = 0; i < x.len; i++) {
+      const int v = nodes[i];
+      const int64_t t = x.first + i;
+      const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1] - 1;
+      int64_t e = e0;
+      while (e < e1 && ctx->pos_pos[e] < h) e++;
+      touter[t] = (int32_t)(e - e0);
+      for (; e < e1; e++) {
+        inner_w.push_back(ctx->pos_w[e]);
+        inner_k.push_back((uint8_t)(ctx->pos_pos[e] - h));
+        inner_i.push_back((uint8_t)i);
+      }
+      tinner[t + 1] = (int32_t)inner_w.size();
+      maxdepth = std::max(maxdepth, ctx->depth[v]);
+    }
+  }
+  // work units of the four chain kernels, chain depth by chain depth
+  std::vector<int4> units;
+  P.rounds.clear();
+  for (size_t c0 = 0; c0 < corder.size();) {
+    size_t c1 = c0;
+    const int cd = hc[corder[c0]].cd;
+    while (c1 < corder.size() && hc[corder[c1]].cd == cd) c1++;
+    LabelPlan::Round rd{};
+    rd.k1b = (int)units.size();
+    for (size_t ci = c0; ci < c1; ci++) {
+      const int passes = std::max(1, (ch[ci].h0 - 1 + 64 * R - 1) / (64 * R));
+      for (int i = 0; i < ch[ci].len; i++)
+        for (int q = 0; q < passes; q++) units.push_back(make_int4((int)ci, i, q * 64 * R, 0));
+    }
+    rd.k1n = (int)units.size() - rd.k1b;
+    rd.k2b = (int)units.size();
+    for (size_t ci = c0; ci < c1; ci++)
+      for (int j0 = 0; j0 < ch[ci].h0 - 1; j0 += 64) units.push_back(make_int4((int)ci, j0, 0, 0));
+    rd.k2n = (int)units.size() - rd.k2b;
+    rd.k3b = (int)units.size();
+    for (size_t ci = c0; ci < c1; ci++)
+      for (int i = 0; i < ch[ci].len; i++) units.push_back(make_int4((int)ci, i, 0, 0));
+    rd.k3n = (int)units.size() - rd.k3b;
+    rd.k4b = (int)units.size();
+    for (size_t ci = c0; ci < c1; ci++) {
+      units.push_back(make_int4((int)ci, 0, 0, 0));
+      rd.lmax = std::max(rd.lmax, ch[ci].len);
+      rd.imax = std::max(rd.imax, tinner[ch[ci].first + ch[ci].len] - tinner[ch[ci].first]);
+    }
+    rd.k4n = (int)units.size() - rd.k4b;
+    P.rounds.push_back(rd);
+    c0 = c1;
+  }
+  P.kcl_b = (int)units.size();  // every chain, for the closure launch
+  for (size_t ci = 0; ci < corder.size(); ci++) units.push_back(make_int4((int)ci, 0, 0, 0));
+  P.kcl_n = (int)corder.size();
+  H2H_TRY(upload(ctx, P.chains, ch.data(), ch.size() * sizeof(Chain)));
+  H2H_TRY(upload(ctx, P.tnode, tnode.data(), tnode.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.touter, touter.data(), touter.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.tinner, tinner.data(), tinner.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.inner_w, inner_w.data(), inner_w.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.inner_k, inner_k.data(), inner_k.size()));
+  H2H_TRY(upload(ctx, P.inner_i, inner_i.data(), inner_i.size()));
+  H2H_TRY(upload(ctx, P.tpath, tpath.data(), tpath.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.cunits, units.data(), units.size() * sizeof(int4)));
+  auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };
+  P.nt = nt;
+  P.tbo = up64(ctx->label_entries);
+  P.tro = up64(P.tbo + (int64_t)maxdepth * nt);
+  P.pro = up64(P.tro + tri);
+  P.dis_entries = P.pro + tri;
+  P.lvl_off.assign(1, 0);
+  P.n_items = P.n_init = 0;
+  return H2H_OK;
+}
+
+h2h_status build_plan(h2h_ctx* ctx, int method, LabelPlan& P) {
+  P.reset();
+  H2H_TRY(method == H2H_METHOD_BATCHED ? plan_batched(ctx, P) : plan_items(ctx, method, P));
   P.method = method;
   return H2H_OK;
 }
 
-void enqueue_plan(const LabelPlan& P, const LabelArgs& a, hipStream_t s) {
+using EventList = std::vector<std::pair<hipEvent_t, hipEvent_t>>;
+
+// Enqueues one build; with `ev`, brackets every launch with a pair of events.
+h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hipStream_t s, EventList* ev) {
+  auto timed = [&](auto&& launch) -> h2h_status {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ev) {
+      H2H_HIP(ctx, hipEventCreate(&e0));
+      H2H_HIP(ctx, hipEventCreate(&e1));
+      H2H_HIP(ctx, hipEventRecord(e0, s));
+    }
+    launch();
+    if (ev) {
+      H2H_HIP(ctx, hipEventRecord(e1, s));
+      ev->emplace_back(e0, e1);
+    }
+    return H2H_OK;
+  };
   if (P.n_init > 0)
-    hipLaunchKernelGGL(label_init_kernel, dim3((unsigned)((P.n_init + 3) / 4)), dim3(256), 0, s, a,
-                       P.init_nodes.as<int32_t>(), P.init_depth.as<int32_t>(), (int)P.n_init);
+    H2H_TRY(timed([&] {
+      hipLaunchKernelGGL(label_init_kernel, dim3((unsigned)((P.n_init + 3) / 4)), dim3(256), 0, s, a,
+                         P.init_nodes.as<int32_t>(), P.init_depth.as<int32_t>(), (int)P.n_init);
+    }));
   const int levels = (int)P.lvl_off.size() - 1;
   for (int l = 0; l < levels; l++) {
     const int cnt = P.lvl_off[l + 1] - P.lvl_off[l];
     if (cnt == 0) continue;
-    launch_items(a, cnt, dim3(P.lvl_block[l]), s, P.items.as<Item>(), P.lvl_off[l], l + 1);
+    H2H_TRY(timed([&] { launch_items(a, cnt, dim3(P.lvl_block[l]), s, P.items.as<Item>(), P.lvl_off[l], l + 1); }));
+  }
+  if (P.method == H2H_METHOD_BATCHED) {
+    ChainArgs c;
+    c.a = a;
+    c.chains = P.chains.as<Chain>();
+    c.tnode = P.tnode.as<int32_t>();
+    c.touter = P.touter.as<int32_t>();
+    c.tinner = P.tinner.as<int32_t>();
+    c.inner_w = P.inner_w.as<int32_t>();
+    c.inner_k = P.inner_k.as<uint8_t>();
+    c.inner_i = P.inner_i.as<uint8_t>();
+    c.tpath = P.tpath.as<int32_t>();
+    c.units = P.cunits.as<int4>();
+    c.tro = P.tro;
+    c.pro = P.pro;
+    c.begin = c.count = 0;
+    if (P.kcl_n > 0) H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b, P.kcl_n, s); }));
+    for (const auto& rd : P.rounds) {
+      const int n[4] = {rd.k1n, rd.k2n, a.l4 ? rd.k3n : 0, rd.k4n};
+      for (int step = 0; step < 4; step++)
+        if (n[step] > 0) H2H_TRY(timed([&] { launch_chain_round(c, rd, s, step); }));
+    }
   }
   if (P.ntasks > 0) {
     TaskArgs t;
@@ -461,8 +1074,36 @@ void enqueue_plan(const LabelPlan& P, const LabelArgs& a, hipStream_t s) {
     t.lvl_depth = P.task_lvl_depth.as<int32_t>();
     t.nodes = P.task_nodes.as<int32_t>();
     t.ntasks = P.ntasks;
-    launch_subtrees(t, s);
+    H2H_TRY(timed([&] { launch_subtrees(t, s); }));
   }
+  return H2H_OK;
+}
+
+// Plan, device buffers and kernel arguments of one build.
+h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, LabelPlan*& plan) {
+  const bool l4 = (kind == H2H_LABEL_L4);
+  if (l4 && ctx->label_entries >= (int64_t)1 << 32)
+    return set_err(ctx, H2H_ERR_OOM, "L4 build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
+  LabelPlan& P = ctx->lplan[method];
+  if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
+  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
+  H2H_TRY(dev_alloc(ctx, ctx->d_dis, std::max<int64_t>(P.dis_entries, 1) * sizeof(int32_t)));
+  if (method == H2H_METHOD_BATCHED) set_chain_lds_limits();
+  a.bfs = ctx->d_bfs.as<int32_t>();
+  a.father = ctx->d_father.as<int32_t>();
+  a.dis_hash = ctx->d_dis_hash.as<int64_t>();
+  a.pos_hash = ctx->d_pos_hash.as<int64_t>();
+  a.pos_pos = ctx->d_pos_pos.as<int32_t>();
+  a.pos_w = ctx->d_pos_w.as<int32_t>();
+  a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
+  a.dis = ctx->d_dis.as<int32_t>();
+  a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
+  a.l4 = l4 ? 1 : 0;
+  a.idx32 = P.dis_entries < ((int64_t)1 << 30) ? 1 : 0;
+  a.tbo = P.tbo;
+  a.nt = P.nt;
+  plan = &P;
+  return H2H_OK;
 }
 
 }  // namespace
@@ -494,32 +1135,29 @@ h2h_status upload_index(h2h_ctx* ctx) {
 h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   hipStream_t s = ctx->stream;
   const bool l4 = (kind == H2H_LABEL_L4);
-  if (l4 && ctx->label_entries >= (int64_t)1 << 32)
-    return set_err(ctx, H2H_ERR_OOM, "L4 build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
-  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
   LabelArgs a;
-  a.bfs = ctx->d_bfs.as<int32_t>();
-  a.father = ctx->d_father.as<int32_t>();
-  a.dis_hash = ctx->d_dis_hash.as<int64_t>();
-  a.pos_hash = ctx->d_pos_hash.as<int64_t>();
-  a.pos_pos = ctx->d_pos_pos.as<int32_t>();
-  a.pos_w = ctx->d_pos_w.as<int32_t>();
-  a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
-  a.dis = ctx->d_dis.as<int32_t>();
-  a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
-  a.l4 = l4 ? 1 : 0;
-  a.idx32 = ctx->label_entries < ((int64_t)1 << 30) ? 1 : 0;
-  LabelPlan& P = ctx->lplan[method];
-  if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
+  LabelPlan* pp = nullptr;
+  H2H_TRY(prepare_build(ctx, kind, method, a, pp));
+  LabelPlan& P = *pp;
+  if (method == H2H_METHOD_BATCHED && P.graph[l4] && P.graph_dis[l4] != ctx->d_dis.p) {
+    (void)hipGraphExecDestroy(P.graph[l4]);
+    P.graph[l4] = nullptr;
+  }
   if (method == H2H_METHOD_BATCHED && !P.graph[l4]) {
-    // capture the whole level sweep once; replays skip per-launch host work
+    // capture the whole build once; replays skip per-launch host work
     hipGraph_t g = nullptr;
     H2H_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    enqueue_plan(P, a, s);
-    H2H_HIP(ctx, hipStreamEndCapture(s, &g));
+    const h2h_status st = enqueue_plan(ctx, P, a, s, nullptr);
+    const hipError_t ce = hipStreamEndCapture(s, &g);
+    if (st != H2H_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (ce != hipSuccess) return hip_err(ctx, ce, "hipStreamEndCapture");
     hipError_t e = hipGraphInstantiate(&P.graph[l4], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return hip_err(ctx, e, "hipGraphInstantiate");
+    P.graph_dis[l4] = ctx->d_dis.p;
   }
   hipEvent_t e0, e1;
   H2H_HIP(ctx, hipEventCreate(&e0));
@@ -528,7 +1166,7 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   if (method == H2H_METHOD_BATCHED) {
     H2H_HIP(ctx, hipGraphLaunch(P.graph[l4], s));
   } else {
-    enqueue_plan(P, a, s);
+    H2H_TRY(enqueue_plan(ctx, P, a, s, nullptr));
   }
   H2H_HIP(ctx, hipGetLastError());
   H2H_HIP(ctx, hipEventRecord(e1, s));
@@ -546,77 +1184,24 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
 
 h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kernel_us, int64_t* launches) {
   hipStream_t s = ctx->stream;
-  const bool l4 = (kind == H2H_LABEL_L4);
-  if (l4 && ctx->label_entries >= (int64_t)1 << 32)
-    return set_err(ctx, H2H_ERR_OOM, "L4 build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
-  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
   LabelArgs a;
-  a.bfs = ctx->d_bfs.as<int32_t>();
-  a.father = ctx->d_father.as<int32_t>();
-  a.dis_hash = ctx->d_dis_hash.as<int64_t>();
-  a.pos_hash = ctx->d_pos_hash.as<int64_t>();
-  a.pos_pos = ctx->d_pos_pos.as<int32_t>();
-  a.pos_w = ctx->d_pos_w.as<int32_t>();
-  a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
-  a.dis = ctx->d_dis.as<int32_t>();
-  a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
-  a.l4 = l4 ? 1 : 0;
-  a.idx32 = ctx->label_entries < ((int64_t)1 << 30) ? 1 : 0;
-  LabelPlan& P = ctx->lplan[method];
-  if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
-  const int levels = (int)P.lvl_off.size() - 1;
-  std::vector<hipEvent_t> ev(2 * levels + 2, nullptr);
-  for (auto& e : ev) H2H_HIP(ctx, hipEventCreate(&e));
-  int64_t nl = 0;
-  if (P.n_init > 0) {
-    H2H_HIP(ctx, hipEventRecord(ev[2 * levels], s));
-    hipLaunchKernelGGL(label_init_kernel, dim3((unsigned)((P.n_init + 3) / 4)), dim3(256), 0, s, a,
-                       P.init_nodes.as<int32_t>(), P.init_depth.as<int32_t>(), (int)P.n_init);
-    H2H_HIP(ctx, hipEventRecord(ev[2 * levels + 1], s));
-  }
-  for (int l = 0; l < levels; l++) {
-    const int cnt = P.lvl_off[l + 1] - P.lvl_off[l];
-    H2H_HIP(ctx, hipEventRecord(ev[2 * l], s));
-    if (cnt > 0) {
-      launch_items(a, cnt, dim3(P.lvl_block[l]), s, P.items.as<Item>(), P.lvl_off[l], l + 1);
-      nl++;
-    }
-    H2H_HIP(ctx, hipEventRecord(ev[2 * l + 1], s));
-  }
-  hipEvent_t t0 = nullptr, t1 = nullptr;
-  if (P.ntasks > 0) {
-    H2H_HIP(ctx, hipEventCreate(&t0));
-    H2H_HIP(ctx, hipEventCreate(&t1));
-    TaskArgs t;
-    t.a = a;
-    t.lvl_base = P.task_lvl_base.as<int32_t>();
-    t.lvl_off = P.task_lvl_off.as<int32_t>();
-    t.lvl_depth = P.task_lvl_depth.as<int32_t>();
-    t.nodes = P.task_nodes.as<int32_t>();
-    t.ntasks = P.ntasks;
-    H2H_HIP(ctx, hipEventRecord(t0, s));
-    launch_subtrees(t, s);
-    H2H_HIP(ctx, hipEventRecord(t1, s));
-    nl++;
-  }
-  H2H_HIP(ctx, hipGetLastError());
-  H2H_HIP(ctx, hipStreamSynchronize(s));
+  LabelPlan* pp = nullptr;
+  H2H_TRY(prepare_build(ctx, kind, method, a, pp));
+  EventList ev;
+  const h2h_status st = enqueue_plan(ctx, *pp, a, s, &ev);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
   double tot = 0;
-  for (int l = 0; l < levels; l++) {
+  for (auto& pr : ev) {
     float ms = 0;
-    H2H_HIP(ctx, hipEventElapsedTime(&ms, ev[2 * l], ev[2 * l + 1]));
-    tot += ms * 1000.0;
+    if (e == hipSuccess && st == H2H_OK && hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) tot += ms * 1000.0;
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
   }
-  if (t0) {
-    float ms = 0;
-    H2H_HIP(ctx, hipEventElapsedTime(&ms, t0, t1));
-    tot += ms * 1000.0;
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
-  }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (st != H2H_OK) return st;
+  if (e != hipSuccess) return hip_err(ctx, e, "label build");
   if (kernel_us) *kernel_us = tot;
-  if (launches) *launches = nl;
+  if (launches) *launches = (int64_t)ev.size();
   ctx->label_kind = kind;
   ctx->labels_ready = true;
   return H2H_OK;

@@ -151,3 +151,20 @@ def test_labels_upload_roundtrip():
             p.build_labels(kind, h2h.PF)
             o.labels(O.SERIAL, l4=(kind == h2h.L4))
             np.testing.assert_array_equal(p.labels()["dis"], o.dis())
+
+
+@pytest.mark.parametrize("task_nodes,chain_nodes", [(1, 1), (16, 3), (64, 128), (300, 7), (10 ** 9, 128)])
+def test_batched_plan_shapes(monkeypatch, task_nodes, chain_nodes):
+    """The batched build at every split between subtree tasks and chains:
+    chains of one node (one chain depth per tree level), short chains that
+    split separators, the default caps, and a tree that is all tasks."""
+    monkeypatch.setenv("H2H_TASK_NODES", str(task_nodes))
+    monkeypatch.setenv("H2H_CHAIN_NODES", str(chain_nodes))
+    for case, C in ((cases.grid64(), 1), (cases.delaunay(30000, 2.6, 5, 7), 2)):
+        o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=4)
+        with p:
+            for kind in (h2h.L4, h2h.L3):
+                _compare_labels(o, p, kind, h2h.BATCHED)
+                for _ in range(2):   # graph replays
+                    p.build_labels(kind, h2h.BATCHED)
+                    np.testing.assert_array_equal(p.labels()["dis"], o.dis())
