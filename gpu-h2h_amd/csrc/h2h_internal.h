@@ -72,7 +72,10 @@ struct LabelPlan {
   std::vector<int32_t> lvl_block;  // threads per block for each level
   DevArray items, init_nodes, init_depth;
   int64_t n_items = 0, n_init = 0;
-  DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // persistent subtree tasks
+  DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // subtree tasks: BFS levels
+  DevArray task_banc, task_tanc, task_info, task_wins, task_dn_row, task_dn_tdep, task_dn_ie;
+  DevArray task_oe_p, task_oe_w, task_oe_ro, task_ie_s, task_ie_w, task_units;  // DFS streams
+  int task_nunits = 0;
   int ntasks = 0;
   int64_t task_nodes_n = 0;
   // BATCHED top part: the tree above the subtree tasks as vertical chains
@@ -97,6 +100,7 @@ struct LabelPlan {
     lvl_block.clear();
     n_items = n_init = 0;
     ntasks = 0;
+    task_nunits = 0;
     task_nodes_n = 0;
     rounds.clear();
     nt = tbo = tro = pro = dis_entries = 0;

@@ -238,39 +238,6 @@ __device__ inline void item_row(const LabelArgs& a, int v, int d, int64_t e0, in
   if ((flags & ITEM_LEAD) && wid == 0 && lane == 0) a.dis[rowv + L] = 0;
 }
 
-// One wavefront computes the columns [c0, c0 + 64R) of node v's row from its
-// whole bag and stores them; the unit with c0 = 0 also writes the self entry.
-template <bool L4, bool IDX32>
-__device__ inline void unit_row(const LabelArgs& a, int v, int d, int c0, int lane) {
-  const int64_t rowv = a.dis_hash[v];
-  const int L = d - 1;
-  const int64_t e0 = a.pos_hash[v];
-  const int nk = (int)(a.pos_hash[v + 1] - 1 - e0);
-  int64_t ancf = 0;
-  if (L4 && d > 1) ancf = a.dis_hash[a.father[v]];
-  if (L4) {
-    const int jend = min(c0 + 64 * R, L);
-    for (int j = c0 + lane; j < jend; j += 64) a.anc[rowv + j] = a.anc[ancf + j];
-    if (c0 == 0 && lane == 0) a.anc[rowv + L] = (uint32_t)rowv;
-  }
-  if (c0 == 0 && lane == 0) a.dis[rowv + L] = 0;
-  if (c0 >= L) return;
-  int32_t acc[R];
-  uint32_t aoff[R];
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    acc[r] = INF32;
-    const int j = c0 + r * 64 + lane;
-    aoff[r] = (L4 && j < L) ? a.anc[ancf + j] : 0u;
-  }
-  accumulate<L4, IDX32>(a, e0, 1, nk, c0, L, lane, aoff, acc);
-#pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int j = c0 + r * 64 + lane;
-    if (j < L) a.dis[rowv + j] = acc[r];
-  }
-}
-
 // Workgroups go to the 8 XCDs round-robin; block b takes item
 // (b % 8) * chunk + b / 8 so that each XCD works through a contiguous run of
 // the level's items (siblings, which read the same ancestor rows, share an L2).
@@ -314,49 +281,6 @@ __global__ void __launch_bounds__(256) label_init_kernel(LabelArgs a, const int3
   if (lane == 0) a.dis[rowv + d - 1] = 0;
 }
 
-// Subtree tasks: a maximal subtree of at most a few hundred nodes depends only
-// on itself and on ancestors finished by earlier launches, so one workgroup
-// walks it level by level with workgroup barriers (its waves share one CU,
-// hence one L1; no device-scope fence is needed between its levels).  The
-// waves take (node, column pass) units of a level round-robin.
-struct TaskArgs {
-  LabelArgs a;
-  const int32_t* lvl_base;   // [ntasks+1] first level of each task
-  const int32_t* lvl_off;    // [levels+1] node offsets of every task level
-  const int32_t* lvl_depth;  // [levels] depth of the level's nodes
-  const int32_t* nodes;      // vertex ids, task by task, level by level
-  int ntasks;
-};
-
-template <bool L4, bool IDX32>
-__global__ void __launch_bounds__(1024) label_subtree_kernel(TaskArgs t) {
-  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = threadIdx.x & 63;
-  const int chunk = (t.ntasks + XCDS - 1) / XCDS;
-  const int task = (int)(blockIdx.x % XCDS) * chunk + (int)(blockIdx.x / XCDS);
-  if (task >= t.ntasks) return;
-  const int l1 = t.lvl_base[task + 1];
-  for (int li = t.lvl_base[task]; li < l1; li++) {
-    const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
-    const int d = t.lvl_depth[li];
-    const int passes = d > 1 ? (d - 1 + 64 * R - 1) / (64 * R) : 1;
-    const int units = (n1 - n0) * passes;
-    for (int u = wid; u < units; u += nw) unit_row<L4, IDX32>(t.a, t.nodes[n0 + u / passes], d, (u % passes) * 64 * R, lane);
-    __syncthreads();
-  }
-}
-
-void launch_subtrees(const TaskArgs& t, hipStream_t s) {
-  const dim3 grid((unsigned)(((t.ntasks + XCDS - 1) / XCDS) * XCDS)), block(1024);
-  if (t.a.idx32) {
-    if (t.a.l4) hipLaunchKernelGGL((label_subtree_kernel<true, true>), grid, block, 0, s, t);
-    else hipLaunchKernelGGL((label_subtree_kernel<false, true>), grid, block, 0, s, t);
-  } else {
-    if (t.a.l4) hipLaunchKernelGGL((label_subtree_kernel<true, false>), grid, block, 0, s, t);
-    else hipLaunchKernelGGL((label_subtree_kernel<false, false>), grid, block, 0, s, t);
-  }
-}
-
-
 // ---- chains (BATCHED top part) ----
 
 struct Chain {
@@ -393,7 +317,7 @@ __device__ inline int xcd_index(int nblocks) {
 inline dim3 xcd_grid(int nblocks) { return dim3((unsigned)(((nblocks + XCDS - 1) / XCDS) * XCDS)); }
 
 // B_i(j) for one (chain node, 512-column pass) per wavefront: the bag entries
-// outside the chain only.  Also writes the node's ancestor row (L4).
+// outside the chain only.
 template <bool L4, bool IDX32>
 __global__ void __launch_bounds__(256) chain_bound_kernel(ChainArgs c) {
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -407,14 +331,6 @@ __global__ void __launch_bounds__(256) chain_bound_kernel(ChainArgs c) {
   const int v = c.tnode[t];
   const int64_t rowv = a.dis_hash[v];
   const int L = ch.h0 - 1;
-  if (L4) {
-    const int f = a.father[c.tnode[ch.first]];
-    const int64_t ancf = f >= 0 ? a.dis_hash[f] : 0;
-    const int jend = min(c0 + 64 * R, L);
-    for (int j = c0 + lane; j < jend; j += 64) a.anc[rowv + j] = a.anc[ancf + j];
-    if (c0 == 0)
-      for (int m = lane; m <= i; m += 64) a.anc[rowv + L + m] = (uint32_t)a.dis_hash[c.tnode[ch.first + m]];
-  }
   if (c0 >= L) return;
   int32_t acc[R];
   uint32_t g[R];
@@ -747,6 +663,227 @@ void set_chain_lds_limits() {
   set_lds_limit(chain_fw_kernel<FW_L4, true>);
   set_lds_limit(chain_fw_kernel<FW_L4, false>);
 }
+// ---- subtree tasks (BATCHED bottom part) ----
+//
+// A task is a maximal subtree of at most smax nodes and TASK_H levels whose
+// root's father lies in a chain.  With r the depth of the root, a task row
+// splits into
+//   outer columns j < r - 1 (ancestors above the task, all in chains):
+//     dis[v][j] = min(B_v(j), min over u in X(v) inside the task of
+//                     w + dis[u][j]),
+//     B_v(j) from the entries above the task, gathers from the transposed
+//     chain rows; one wavefront per (task, 64 columns) walks the task in DFS
+//     preorder, so the in-task ancestors' values of its columns are exactly
+//     a stack kept in LDS (task_outer_kernel);
+//   inner columns r - 1 .. depth(v) - 2 (ancestors inside the task): one
+//     workgroup per task, level by level (task_inner_kernel).
+constexpr int TASK_H = 32;  // task height cap: the LDS stack depth
+constexpr int WIN_N = 16;   // nodes per window of the DFS walk
+constexpr int WIN_E = 64;   // inner entries per window (one per lane)
+
+struct TaskInfo {
+  int32_t dfs;     // first node in the DFS-order arrays
+  int32_t nn;      // nodes
+  int32_t win;     // first window
+  int32_t nwin;    // windows
+  int32_t r;       // depth of the root: the outer columns are [0, r - 1)
+  int32_t path;    // tpath offset of the chain holding the root's father
+  int32_t cfirst;  // chain index of that chain's head
+  int32_t ch0;     // depth of that chain's head
+};
+
+struct TaskWin {   // consecutive DFS nodes: at most WIN_N nodes and WIN_E inner entries
+  int32_t k0, nn;  // first node (global DFS index), nodes
+  int32_t oe, no;  // outer entries
+  int32_t ie, ni;  // inner entries
+  int32_t pad[2];
+};
+
+struct TaskArgs {
+  LabelArgs a;
+  const TaskInfo* tasks;
+  const TaskWin* wins;
+  const int64_t* dn_row;   // DFS nodes: row offset
+  const int32_t* dn_tdep;  // task depth (the stack slot)
+  const int32_t* dn_ie;    // first inner entry
+  const int32_t* oe_p;     // outer entries: position, weight,
+  const int32_t* oe_w;
+  const int64_t* oe_ro;    //   row offset of the bag member | node within the window << 56
+  const uint8_t* ie_s;     // inner entries: task depth of the bag member, weight
+  const int32_t* ie_w;
+  const int32_t* tpath;
+  const int4* units;       // task_outer_kernel: (task, first column)
+  int nunits;
+  const int32_t* lvl_base;   // [ntasks+1] first level of each task
+  const int32_t* lvl_off;    // node offsets of every task level (BFS order)
+  const int32_t* lvl_depth;  // depth of the level's nodes
+  const int32_t* bnodes;     // vertices, task by task, level by level
+  const int32_t* banc;       // per BFS node: offset in tanc of its in-task ancestors' rows
+  const int64_t* tanc;
+  int ntasks;
+};
+
+__device__ inline int64_t readlane64(int64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+template <bool L4, bool IDX32>
+__global__ void __launch_bounds__(256) task_outer_kernel(TaskArgs t) {
+  __shared__ int32_t lds_stack[4][TASK_H * 64];
+  __shared__ int32_t lds_b[4][WIN_N * 64];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u = xcd_index((t.nunits + 3) >> 2) * 4 + wid;
+  if (u >= t.nunits) return;  // whole wavefronts; no workgroup barriers below
+  int32_t* stack = lds_stack[wid];
+  int32_t* Bw = lds_b[wid];
+  const LabelArgs& a = t.a;
+  const int4 un = t.units[u];
+  const TaskInfo T = t.tasks[un.x];
+  const int L = T.r - 1;
+  const int j = un.y + lane;
+  const int jc = min(j, L - 1);  // clamped: every address stays valid
+  uint32_t ta = 0;               // chain index of a_j
+  if (L4) ta = jc < T.ch0 - 1 ? (uint32_t)t.tpath[T.path + jc] : (uint32_t)(T.cfirst + jc - (T.ch0 - 1));
+  for (int wi = 0; wi < T.nwin; wi++) {
+    const TaskWin W = t.wins[T.win + wi];
+    // the window's node and inner-entry metadata, one per lane
+    int64_t nrow = 0;
+    int ntd = 0, nie = W.ni, is = 0, iw = 0;
+    if (lane < W.nn) {
+      nrow = t.dn_row[W.k0 + lane];
+      ntd = t.dn_tdep[W.k0 + lane];
+      nie = t.dn_ie[W.k0 + lane] - W.ie;
+    }
+    if (lane < W.ni) {
+      is = t.ie_s[W.ie + lane];
+      iw = t.ie_w[W.ie + lane];
+    }
+    for (int q = 0; q < W.nn; q++) Bw[q * 64 + lane] = INF32;
+    // B of the window's nodes: outer entries, eight loads in flight
+    for (int e0 = 0; e0 < W.no; e0 += 64) {
+      int mp = 0, mw = 0;
+      int64_t mro = 0;
+      if (e0 + lane < W.no) {
+        mp = t.oe_p[W.oe + e0 + lane];
+        mw = t.oe_w[W.oe + e0 + lane];
+        mro = t.oe_ro[W.oe + e0 + lane];
+      }
+      const int kn = min(64, W.no - e0);
+      for (int q0 = 0; q0 < kn; q0 += 8) {
+        int32_t v[8], wv[8], qn[8];
+        bool use[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+          const int e = min(q0 + r, kn - 1);
+          const int p = __builtin_amdgcn_readlane(mp, e);
+          wv[r] = __builtin_amdgcn_readlane(mw, e);
+          const int64_t ro_q = readlane64(mro, e);
+          qn[r] = (int)((uint64_t)ro_q >> 56);
+          const uint64_t ro = (uint64_t)ro_q & 0x00ffffffffffffffull;
+          use[r] = (q0 + r < kn) && (L4 || jc <= p);
+          const uint64_t idx = L4 ? ((jc <= p) ? ro + jc : (uint64_t)a.tbo + (uint64_t)(uint32_t)p * (uint64_t)a.nt + ta)
+                                  : ro + min(jc, p);
+          v[r] = ld<IDX32>(a.dis, idx);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+          int32_t* b = &Bw[qn[r] * 64 + lane];
+          const int32_t old = *b;
+          *b = use[r] ? min(old, wv[r] + v[r]) : old;
+        }
+      }
+    }
+    // walk the window's nodes: in-task bag members are on the stack
+    for (int q = 0; q < W.nn; q++) {
+      int32_t x = Bw[q * 64 + lane];
+      const int i0 = __builtin_amdgcn_readlane(nie, q), i1 = __builtin_amdgcn_readlane(nie, q + 1);
+      for (int i = i0; i < i1; i++)
+        x = min(x, __builtin_amdgcn_readlane(iw, i) + stack[__builtin_amdgcn_readlane(is, i) * 64 + lane]);
+      stack[__builtin_amdgcn_readlane(ntd, q) * 64 + lane] = x;
+      const int64_t row = readlane64(nrow, q);
+      if (j < L) a.dis[row + j] = x;
+    }
+  }
+}
+
+template <bool L4, bool IDX32>
+__global__ void __launch_bounds__(256) task_inner_kernel(TaskArgs t) {
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int task = xcd_index(t.ntasks);
+  if (task >= t.ntasks) return;  // whole workgroup
+  const LabelArgs& a = t.a;
+  const int r = t.tasks[task].r;
+  const int l1 = t.lvl_base[task + 1];
+  for (int li = t.lvl_base[task]; li < l1; li++) {
+    const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
+    const int d = t.lvl_depth[li] - r;  // task depth: inner columns r-1 .. r-2+d, self at r-1+d
+    for (int k = n0 + wid; k < n1; k += 4) {
+      const int v = t.bnodes[k];
+      const int64_t rowv = a.dis_hash[v];
+      if (d > 0) {
+        const int j = r - 1 + min(lane, d - 1);
+        const int64_t arow = t.tanc[t.banc[k] + min(lane, d - 1)];  // a_j: in-task ancestor at task depth lane
+        const int64_t e0 = a.pos_hash[v];
+        const int nk = (int)(a.pos_hash[v + 1] - 1 - e0);
+        int32_t x = INF32;
+        for (int kb = 0; kb < nk; kb += 64) {
+          int mp = 0, mw = 0;
+          int64_t mro = 0;
+          if (kb + lane < nk) {
+            mp = a.pos_pos[e0 + kb + lane];
+            mw = a.pos_w[e0 + kb + lane];
+            mro = a.pos_rowoff[e0 + kb + lane];
+          }
+          const int kn = min(64, nk - kb);
+          for (int q0 = 0; q0 < kn; q0 += 4) {
+            int32_t y[4], wv[4];
+            bool use[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              const int e = min(q0 + q, kn - 1);
+              const int p = __builtin_amdgcn_readlane(mp, e);
+              wv[q] = __builtin_amdgcn_readlane(mw, e);
+              const int64_t ro = readlane64(mro, e);
+              use[q] = (q0 + q < kn) && (L4 || j <= p);
+              y[q] = ld<IDX32>(a.dis, L4 ? ((j <= p) ? (uint64_t)ro + j : (uint64_t)arow + p) : (uint64_t)ro + min(j, p));
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) x = use[q] ? min(x, wv[q] + y[q]) : x;
+          }
+        }
+        if (lane < d) a.dis[rowv + r - 1 + lane] = x;
+      }
+      if (lane == 0) a.dis[rowv + r - 1 + d] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+void launch_tasks(const TaskArgs& t, hipStream_t s, int step) {
+  const bool l4 = t.a.l4 != 0, i32 = t.a.idx32 != 0;
+  if (step == 0) {
+    const dim3 grid = xcd_grid((t.nunits + 3) / 4);
+    if (i32) {
+      if (l4) hipLaunchKernelGGL((task_outer_kernel<true, true>), grid, dim3(256), 0, s, t);
+      else hipLaunchKernelGGL((task_outer_kernel<false, true>), grid, dim3(256), 0, s, t);
+    } else {
+      if (l4) hipLaunchKernelGGL((task_outer_kernel<true, false>), grid, dim3(256), 0, s, t);
+      else hipLaunchKernelGGL((task_outer_kernel<false, false>), grid, dim3(256), 0, s, t);
+    }
+  } else {
+    const dim3 grid = xcd_grid(t.ntasks);
+    if (i32) {
+      if (l4) hipLaunchKernelGGL((task_inner_kernel<true, true>), grid, dim3(256), 0, s, t);
+      else hipLaunchKernelGGL((task_inner_kernel<false, true>), grid, dim3(256), 0, s, t);
+    } else {
+      if (l4) hipLaunchKernelGGL((task_inner_kernel<true, false>), grid, dim3(256), 0, s, t);
+      else hipLaunchKernelGGL((task_inner_kernel<false, false>), grid, dim3(256), 0, s, t);
+    }
+  }
+}
+
 }  // namespace
 
 #define H2H_TRY(x)               \
@@ -813,54 +950,25 @@ h2h_status plan_items(h2h_ctx* ctx, int method, LabelPlan& P) {
 
 h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   const int n = ctx->n;
-  // maximal subtrees of at most smax nodes become persistent tasks
+  // maximal subtrees of at most smax nodes and TASK_H levels become tasks
   std::vector<int32_t> task_of(n, -1);
   const int64_t smax = env_int("H2H_TASK_NODES", std::max<int64_t>(64, n / (2LL * ctx->num_cus)), 1, INT32_MAX);
   const int chain_max = (int)env_int("H2H_CHAIN_NODES", CHAIN_MAX, 1, CHAIN_MAX);
+  int ntasks = 0;
   {
     std::vector<int64_t> sub(n, 1);
+    std::vector<int32_t> hgt(n, 1);
     for (int i = n - 1; i > 0; i--) {
       const int v = ctx->bfs[i];
       sub[ctx->father[v]] += sub[v];
+      hgt[ctx->father[v]] = std::max(hgt[ctx->father[v]], hgt[v] + 1);
     }
-    int ntasks = 0;
     for (int i = 0; i < n; i++) {
       const int v = ctx->bfs[i];
       const int f = ctx->father[v];
       if (f >= 0 && task_of[f] >= 0) task_of[v] = task_of[f];
-      else if (sub[v] <= smax && f >= 0) task_of[v] = ntasks++;
+      else if (f >= 0 && sub[v] <= smax && hgt[v] <= TASK_H) task_of[v] = ntasks++;
     }
-    std::vector<int32_t> order;  // task nodes, task by task, BFS order inside
-    std::vector<int32_t> cnt(ntasks + 1, 0);
-    for (int i = 0; i < n; i++)
-      if (task_of[ctx->bfs[i]] >= 0) cnt[task_of[ctx->bfs[i]] + 1]++;
-    for (int t = 0; t < ntasks; t++) cnt[t + 1] += cnt[t];
-    order.assign(cnt[ntasks], 0);
-    {
-      std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1);
-      for (int i = 0; i < n; i++) {
-        const int v = ctx->bfs[i];
-        if (task_of[v] >= 0) order[fill[task_of[v]]++] = v;
-      }
-    }
-    // levels of every task: start offsets into `order` and their depth
-    std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth;
-    for (int t = 0; t < ntasks; t++) {
-      for (int k = cnt[t]; k < cnt[t + 1]; k++) {
-        if (k == cnt[t] || ctx->depth[order[k]] != ctx->depth[order[k - 1]]) {
-          starts.push_back(k);
-          lvl_depth.push_back(ctx->depth[order[k]]);
-        }
-      }
-      lvl_base.push_back((int32_t)lvl_depth.size());
-    }
-    starts.push_back((int32_t)order.size());
-    H2H_TRY(upload(ctx, P.task_lvl_base, lvl_base.data(), lvl_base.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_lvl_off, starts.data(), starts.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_lvl_depth, lvl_depth.data(), lvl_depth.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_nodes, order.data(), order.size() * sizeof(int32_t)));
-    P.ntasks = ntasks;
-    P.task_nodes_n = (int64_t)order.size();
   }
 
   // chains over the other nodes, built in BFS order (fathers first): a node
@@ -997,6 +1105,133 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   H2H_TRY(upload(ctx, P.inner_i, inner_i.data(), inner_i.size()));
   H2H_TRY(upload(ctx, P.tpath, tpath.data(), tpath.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.cunits, units.data(), units.size() * sizeof(int4)));
+
+  // tasks: BFS levels (inner columns), DFS streams and windows (outer columns)
+  {
+    std::vector<std::vector<int32_t>> members(ntasks);  // BFS order inside each task
+    for (int i = 0; i < n; i++)
+      if (task_of[ctx->bfs[i]] >= 0) members[task_of[ctx->bfs[i]]].push_back(ctx->bfs[i]);
+    std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth, bnodes, banc;
+    std::vector<int64_t> tanc;
+    std::vector<TaskInfo> infos(ntasks);
+    std::vector<TaskWin> wins;
+    std::vector<int64_t> dn_row, oe_ro;
+    std::vector<int32_t> dn_tdep, dn_ie, oe_p, oe_w, ie_w;
+    std::vector<uint8_t> ie_s;
+    std::vector<int4> tunits;
+    std::vector<int32_t> local(n, -1);
+    for (int t = 0; t < ntasks; t++) {
+      const auto& mem = members[t];
+      const int rt = mem[0];
+      const int r = ctx->depth[rt];
+      for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
+      // BFS levels and in-task ancestor rows (phase B)
+      for (size_t k = 0; k < mem.size(); k++) {
+        const int v = mem[k];
+        if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
+          starts.push_back((int32_t)bnodes.size());
+          lvl_depth.push_back(ctx->depth[v]);
+        }
+        bnodes.push_back(v);
+        banc.push_back((int32_t)tanc.size());
+        const int d = ctx->depth[v] - r;
+        const size_t at = tanc.size();
+        tanc.resize(at + d);
+        int x = v;
+        for (int q = d - 1; q >= 0; q--) {
+          x = ctx->father[x];
+          tanc[at + q] = ctx->dis_hash[x];
+        }
+      }
+      lvl_base.push_back((int32_t)lvl_depth.size());
+      // DFS preorder
+      std::vector<std::vector<int32_t>> kids(mem.size());
+      for (size_t k = 1; k < mem.size(); k++) kids[local[ctx->father[mem[k]]]].push_back((int32_t)k);
+      std::vector<int32_t> dfs, st(1, 0);
+      while (!st.empty()) {
+        const int k = st.back();
+        st.pop_back();
+        dfs.push_back(k);
+        for (auto it = kids[k].rbegin(); it != kids[k].rend(); ++it) st.push_back(*it);
+      }
+      TaskInfo& I = infos[t];
+      std::memset(&I, 0, sizeof(I));
+      I.dfs = (int32_t)dn_row.size();
+      I.nn = (int32_t)mem.size();
+      I.win = (int32_t)wins.size();
+      I.r = r;
+      {
+        const int f = ctx->father[rt];
+        const Chain& fc = ch[cpos[chain_of[f]]];
+        I.path = fc.path;
+        I.cfirst = fc.first;
+        I.ch0 = fc.h0;
+      }
+      TaskWin W;
+      std::memset(&W, 0, sizeof(W));
+      auto close_win = [&]() {
+        if (W.nn > 0) wins.push_back(W);
+        std::memset(&W, 0, sizeof(W));
+        W.k0 = (int32_t)dn_row.size();
+        W.oe = (int32_t)oe_p.size();
+        W.ie = (int32_t)ie_w.size();
+      };
+      close_win();
+      for (int k : dfs) {
+        const int v = mem[k];
+        const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1] - 1;
+        int no = 0, ni = 0;
+        for (int64_t e = e0; e < e1; e++) (ctx->pos_pos[e] < r - 1 ? no : ni)++;
+        if (W.nn == WIN_N || (W.nn > 0 && (W.ni + ni > WIN_E || W.no + no > 64))) close_win();
+        if (ni > WIN_E) return set_err(ctx, H2H_ERR_INPUT, "task plan: node with more in-task bag members than a window holds");
+        const int q = W.nn;
+        dn_row.push_back(ctx->dis_hash[v]);
+        dn_tdep.push_back(ctx->depth[v] - r);
+        dn_ie.push_back((int32_t)ie_w.size());
+        for (int64_t e = e0; e < e1; e++) {
+          const int pp = ctx->pos_pos[e];
+          if (pp < r - 1) {
+            oe_p.push_back(pp);
+            oe_w.push_back(ctx->pos_w[e]);
+            oe_ro.push_back(ctx->dis_hash[ctx->pos_id[e]] | ((int64_t)q << 56));
+          } else {
+            ie_s.push_back((uint8_t)(pp - (r - 1)));
+            ie_w.push_back(ctx->pos_w[e]);
+          }
+        }
+        W.nn++;
+        W.no += no;
+        W.ni += ni;
+      }
+      close_win();
+      I.nwin = (int32_t)wins.size() - I.win;
+      for (int j0 = 0; j0 < r - 1; j0 += 64) tunits.push_back(make_int4(t, j0, 0, 0));
+      for (int v : mem) local[v] = -1;
+    }
+    starts.push_back((int32_t)bnodes.size());
+    if (ctx->label_entries >= ((int64_t)1 << 56))
+      return set_err(ctx, H2H_ERR_OOM, "task plan: row offsets above 2^56");
+    H2H_TRY(upload(ctx, P.task_lvl_base, lvl_base.data(), lvl_base.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_lvl_off, starts.data(), starts.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_lvl_depth, lvl_depth.data(), lvl_depth.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_nodes, bnodes.data(), bnodes.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_banc, banc.data(), banc.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_tanc, tanc.data(), tanc.size() * sizeof(int64_t)));
+    H2H_TRY(upload(ctx, P.task_info, infos.data(), infos.size() * sizeof(TaskInfo)));
+    H2H_TRY(upload(ctx, P.task_wins, wins.data(), wins.size() * sizeof(TaskWin)));
+    H2H_TRY(upload(ctx, P.task_dn_row, dn_row.data(), dn_row.size() * sizeof(int64_t)));
+    H2H_TRY(upload(ctx, P.task_dn_tdep, dn_tdep.data(), dn_tdep.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_dn_ie, dn_ie.data(), dn_ie.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_oe_p, oe_p.data(), oe_p.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_oe_w, oe_w.data(), oe_w.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_oe_ro, oe_ro.data(), oe_ro.size() * sizeof(int64_t)));
+    H2H_TRY(upload(ctx, P.task_ie_s, ie_s.data(), ie_s.size()));
+    H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_units, tunits.data(), tunits.size() * sizeof(int4)));
+    P.ntasks = ntasks;
+    P.task_nodes_n = (int64_t)bnodes.size();
+    P.task_nunits = (int)tunits.size();
+  }
   auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };
   P.nt = nt;
   P.tbo = up64(ctx->label_entries);
@@ -1069,12 +1304,28 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
   if (P.ntasks > 0) {
     TaskArgs t;
     t.a = a;
+    t.tasks = P.task_info.as<TaskInfo>();
+    t.wins = P.task_wins.as<TaskWin>();
+    t.dn_row = P.task_dn_row.as<int64_t>();
+    t.dn_tdep = P.task_dn_tdep.as<int32_t>();
+    t.dn_ie = P.task_dn_ie.as<int32_t>();
+    t.oe_p = P.task_oe_p.as<int32_t>();
+    t.oe_w = P.task_oe_w.as<int32_t>();
+    t.oe_ro = P.task_oe_ro.as<int64_t>();
+    t.ie_s = P.task_ie_s.as<uint8_t>();
+    t.ie_w = P.task_ie_w.as<int32_t>();
+    t.tpath = P.tpath.as<int32_t>();
+    t.units = P.task_units.as<int4>();
+    t.nunits = P.task_nunits;
     t.lvl_base = P.task_lvl_base.as<int32_t>();
     t.lvl_off = P.task_lvl_off.as<int32_t>();
     t.lvl_depth = P.task_lvl_depth.as<int32_t>();
-    t.nodes = P.task_nodes.as<int32_t>();
+    t.bnodes = P.task_nodes.as<int32_t>();
+    t.banc = P.task_banc.as<int32_t>();
+    t.tanc = P.task_tanc.as<int64_t>();
     t.ntasks = P.ntasks;
-    H2H_TRY(timed([&] { launch_subtrees(t, s); }));
+    if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
+    H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
   }
   return H2H_OK;
 }
@@ -1082,11 +1333,12 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
 // Plan, device buffers and kernel arguments of one build.
 h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, LabelPlan*& plan) {
   const bool l4 = (kind == H2H_LABEL_L4);
-  if (l4 && ctx->label_entries >= (int64_t)1 << 32)
-    return set_err(ctx, H2H_ERR_OOM, "L4 build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
+  if (l4 && method != H2H_METHOD_BATCHED && ctx->label_entries >= (int64_t)1 << 32)
+    return set_err(ctx, H2H_ERR_OOM, "L4 level build keeps 32-bit ancestor row offsets: index above 2^32 label entries");
   LabelPlan& P = ctx->lplan[method];
   if (P.method != method) H2H_TRY(build_plan(ctx, method, P));
-  if (l4) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
+  const bool anc = l4 && method != H2H_METHOD_BATCHED;  // ancestor rows: the level kernels only
+  if (anc) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
   H2H_TRY(dev_alloc(ctx, ctx->d_dis, std::max<int64_t>(P.dis_entries, 1) * sizeof(int32_t)));
   if (method == H2H_METHOD_BATCHED) set_chain_lds_limits();
   a.bfs = ctx->d_bfs.as<int32_t>();
@@ -1097,7 +1349,7 @@ h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, Label
   a.pos_w = ctx->d_pos_w.as<int32_t>();
   a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
   a.dis = ctx->d_dis.as<int32_t>();
-  a.anc = l4 ? ctx->d_anc.as<uint32_t>() : nullptr;
+  a.anc = anc ? ctx->d_anc.as<uint32_t>() : nullptr;
   a.l4 = l4 ? 1 : 0;
   a.idx32 = P.dis_entries < ((int64_t)1 << 30) ? 1 : 0;
   a.tbo = P.tbo;
