@@ -74,8 +74,8 @@ struct LabelPlan {
   int64_t n_items = 0, n_init = 0;
   DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // subtree tasks: BFS levels
   DevArray task_banc, task_tanc, task_info, task_wins, task_dn_row, task_dn_tdep, task_dn_ie;
-  DevArray task_oe_p, task_oe_w, task_oe_ro, task_ie_s, task_ie_w, task_units;  // DFS streams
-  int task_nunits = 0;
+  DevArray task_ie_s, task_ie_w, task_units, bunits;  // DFS walk; bound units of chains and tasks
+  int task_nunits = 0, tbound_b = 0, tbound_n = 0;
   int ntasks = 0;
   int64_t task_nodes_n = 0;
   // BATCHED top part: the tree above the subtree tasks as vertical chains
@@ -84,7 +84,7 @@ struct LabelPlan {
     int k1b, k1n, k2b, k2n, k3b, k3n, k4b, k4n, lmax, imax;
   };
   std::vector<Round> rounds;
-  int kcl_b = 0, kcl_n = 0;  // units of the closure launch (every chain)
+  int kcl_b = 0, kcl_n = 0, kcl_lmax = 0;  // units of the closure launch (every chain)
   int64_t nt = 0;           // nodes in chains
   int64_t tbo = 0;          // offset of the transposed chain rows inside dis
   int64_t tro = 0;          // offset of the t_i(m) triangles (scratch) inside dis
@@ -100,7 +100,7 @@ struct LabelPlan {
     lvl_block.clear();
     n_items = n_init = 0;
     ntasks = 0;
-    task_nunits = 0;
+    task_nunits = tbound_b = tbound_n = 0;
     task_nodes_n = 0;
     rounds.clear();
     nt = tbo = tro = pro = dis_entries = 0;

@@ -253,7 +253,8 @@ __global__ void __launch_bounds__(256) label_items_kernel(LabelArgs a, const Ite
   const Item it = items[begin + idx];
   const int v = a.bfs[it.node];
   const int64_t e0 = a.pos_hash[v] + it.first;
-  item_row<L4, IDX32>(a, v, d, e0, e0 + it.count, it.flags, threadIdx.x >> 6, blockDim.x >> 6, threadIdx.x & 63,
+  item_row<L4, IDX32>(a, v, d, e0, e0 + it.count, it.flags, __builtin_amdgcn_readfirstlane(threadIdx.x >> 6),
+                      blockDim.x >> 6, threadIdx.x & 63,
                       comb);
 }
 
@@ -283,6 +284,8 @@ __global__ void __launch_bounds__(256) label_init_kernel(LabelArgs a, const int3
 
 // ---- chains (BATCHED top part) ----
 
+struct BoundUnit;
+
 struct Chain {
   int32_t first;  // chain index t of the head; a chain's nodes have consecutive indices
   int32_t len;    // nodes (at most 128)
@@ -303,6 +306,7 @@ struct ChainArgs {
   const uint8_t* inner_i;  // index in the chain of the node whose bag holds the entry
   const int32_t* tpath;
   const int4* units;       // per-kernel work units
+  const BoundUnit* bunits;  // bound units (chain rounds, then the tasks)
   int64_t tro;             // t_i(m) triangles (scratch inside dis)
   int64_t pro;             // closure triangles P (scratch inside dis)
   int begin, count;        // units of this launch
@@ -316,35 +320,49 @@ __device__ inline int xcd_index(int nblocks) {
 
 inline dim3 xcd_grid(int nblocks) { return dim3((unsigned)(((nblocks + XCDS - 1) / XCDS) * XCDS)); }
 
-// B_i(j) for one (chain node, 512-column pass) per wavefront: the bag entries
-// outside the chain only.
+// Bounds from the bag entries above a node's chain or task: one unit =
+// columns [c0, c0 + 512) of the outer columns [0, L) of vertex v, from its
+// first `nouter` bag entries (positions below L).  The ancestor a_j of column
+// j has chain index j < ch0 - 1 ? tpath[path + j] : cfirst + j - (ch0 - 1)
+// (the chain holding v's first ancestor above the chain or task).
+struct BoundUnit {
+  int32_t v, c0, L, nouter;
+  int32_t path, cfirst, ch0, pad;
+};
+
 template <bool L4, bool IDX32>
-__global__ void __launch_bounds__(256) chain_bound_kernel(ChainArgs c) {
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int u = xcd_index((c.count + 3) >> 2) * 4 + wid;
-  if (u >= c.count) return;  // whole wavefronts; no barriers below
-  const LabelArgs& a = c.a;
-  const int4 un = c.units[c.begin + u];
-  const Chain ch = c.chains[un.x];
-  const int i = un.y, c0 = un.z;
-  const int t = ch.first + i;
-  const int v = c.tnode[t];
-  const int64_t rowv = a.dis_hash[v];
-  const int L = ch.h0 - 1;
-  if (c0 >= L) return;
+__global__ void __launch_bounds__(256) bound_kernel(LabelArgs a, const BoundUnit* __restrict__ units,
+                                                  const int32_t* __restrict__ tpath, int begin, int count) {
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int u = xcd_index((count + 3) >> 2) * 4 + wid;
+  if (u >= count) return;  // whole wavefronts; no barriers below
+  const BoundUnit b = units[begin + u];
+  const int64_t rowv = a.dis_hash[b.v];
   int32_t acc[R];
   uint32_t g[R];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     acc[r] = INF32;
-    const int j = c0 + r * 64 + lane;
-    g[r] = (L4 && j < L) ? (uint32_t)c.tpath[ch.path + j] : 0u;
+    const int jj = min(b.c0 + r * 64 + lane, b.L - 1);
+    g[r] = L4 ? (jj < b.ch0 - 1 ? (uint32_t)tpath[b.path + jj] : (uint32_t)(b.cfirst + jj - (b.ch0 - 1))) : 0u;
   }
-  accumulate<L4, IDX32, true>(a, a.pos_hash[v], 1, c.touter[t], c0, L, lane, g, acc);
+  accumulate<L4, IDX32, true>(a, a.pos_hash[b.v], 1, b.nouter, b.c0, b.L, lane, g, acc);
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const int j = c0 + r * 64 + lane;
-    if (j < L) a.dis[rowv + j] = acc[r];
+    const int j = b.c0 + r * 64 + lane;
+    if (j < b.L) a.dis[rowv + j] = acc[r];
+  }
+}
+
+void launch_bound(const LabelArgs& a, const BoundUnit* units, const int32_t* tpath, int begin, int count,
+                  hipStream_t s) {
+  const dim3 grid = xcd_grid((count + 3) / 4);
+  if (a.idx32) {
+    if (a.l4) hipLaunchKernelGGL((bound_kernel<true, true>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
+    else hipLaunchKernelGGL((bound_kernel<false, true>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
+  } else {
+    if (a.l4) hipLaunchKernelGGL((bound_kernel<true, false>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
+    else hipLaunchKernelGGL((bound_kernel<false, false>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
   }
 }
 
@@ -377,7 +395,7 @@ struct ChainRows {
 template <bool IDX32>
 __global__ void __launch_bounds__(256) chain_apply_kernel(ChainArgs c) {
   extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wid = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int u = xcd_index(c.count);
   if (u >= c.count) return;  // whole workgroup
   const LabelArgs& a = c.a;
@@ -438,7 +456,7 @@ __global__ void __launch_bounds__(256) chain_apply_kernel(ChainArgs c) {
 // diagonal 0 into row i of the chain's triangle scratch.
 template <bool IDX32>
 __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int u = xcd_index((c.count + 3) >> 2) * 4 + wid;
   if (u >= c.count) return;
   const LabelArgs& a = c.a;
@@ -489,22 +507,22 @@ __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
   if (lane + 64 <= i) a.dis[trow + lane + 64] = lane + 64 == i ? 0 : x1;
 }
 
-// Floyd-Warshall over one chain's nodes in a workgroup (256 threads, chains
-// of at most FW_N nodes): thread (column m, row block g) keeps the 64 cells
-// T[g * 64 + r][m] in registers; step k publishes row k (and, when the
-// matrix is not symmetric, column k) through LDS and every thread relaxes its
-// cells through k.  Cells past the chain stay INF (saturating adds).
+// Floyd-Warshall over one chain's nodes in a workgroup of fw_threads(S)
+// threads, for chains of at most S nodes (S = 32, 64, 128): thread (column m,
+// row group g) keeps the cells T[g * RPT + r][m], r < RPT, in registers;
+// step k publishes row k (and, when the matrix is not symmetric, column k)
+// through LDS and every thread relaxes its cells through k.  Cells past the
+// chain stay INF (saturating adds).
 //   FW_CLOSURE  P = closure of the inner entries as a DAG (paths that go up
 //               the chain), lower triangle to the P scratch;
 //   FW_L3       the same closure is the L3 triangle: written to the rows;
 //   FW_L4       the symmetric graph of inner entries and t_i(m) edges: its
 //               all-pairs distances are the L4 triangle, written to the rows.
-// LDS: the matrix [FW_N][FW_LD] for set-up and output, row and column buffers.
-constexpr int FW_N = 128;
-constexpr int FW_RPT = 64;
-constexpr int FW_LD = FW_N + 1;  // odd stride: column accesses spread over the banks
+// LDS: the matrix [S][S + 1] for set-up and output, row and column buffers.
 enum { FW_CLOSURE = 0, FW_L3 = 1, FW_L4 = 2 };
-size_t fw_lds() { return (size_t)(FW_N * FW_LD + 4 * FW_N) * 4; }
+size_t fw_lds(int S) { return (size_t)(S * (S + 1) + 4 * S) * 4; }
+__host__ __device__ constexpr int fw_threads(int S) { return S == 128 ? 1024 : S == 64 ? 512 : 256; }
+int fw_size(int lmax) { return lmax <= 32 ? 32 : lmax <= 64 ? 64 : 128; }
 
 // row i of a triangle stored row by row: the largest i with i (i + 1) / 2 <= q
 __device__ inline int tri_row(int q) {
@@ -514,11 +532,14 @@ __device__ inline int tri_row(int q) {
   return i;
 }
 
-template <int MODE, bool IDX32>
-__global__ void __launch_bounds__(256) chain_fw_kernel(ChainArgs c) {
+template <int MODE, int S, bool IDX32>
+__global__ void __launch_bounds__(fw_threads(S)) chain_fw_kernel(ChainArgs c) {
   constexpr bool SYM = MODE == FW_L4;
+  constexpr int NT = fw_threads(S), G = NT / S, RPT = S / G, LD = S + 1;
+  constexpr int KB = RPT < 8 ? RPT : 8;  // steps unrolled per block: the published register is static
   extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, m = tid & (FW_N - 1), g = tid >> 7, lane = tid & 63;
+  const int tid = threadIdx.x, m = tid % S, lane = tid & 63;
+  const int g = S >= 64 ? __builtin_amdgcn_readfirstlane(tid / S) : tid / S;  // uniform per wavefront when S >= 64
   const int u = xcd_index(c.count);
   if (u >= c.count) return;  // whole workgroup
   const LabelArgs& a = c.a;
@@ -526,74 +547,82 @@ __global__ void __launch_bounds__(256) chain_fw_kernel(ChainArgs c) {
   const int n = ch.len, h = ch.h0 - 1;
   const int tot = n * (n + 1) / 2;
   int32_t* M = lds;
-  int32_t* Rb = lds + FW_N * FW_LD;  // [2][FW_N] row k
-  int32_t* Cb = Rb + 2 * FW_N;       // [2][FW_N] column k
-  for (int q = tid; q < FW_N * FW_LD; q += 256) M[q] = INF32;
+  int32_t* Rb = lds + S * LD;  // [2][S] row k
+  int32_t* Cb = Rb + 2 * S;    // [2][S] column k
+  for (int q = tid; q < S * LD; q += NT) M[q] = INF32;
   __syncthreads();
   if (SYM) {
     // t_i(m) edges and the diagonal from the cross scratch
     const uint64_t tb = (uint64_t)c.tro + ch.tri;
-    for (int q = tid; q < tot; q += 256) {
+    for (int q = tid; q < tot; q += NT) {
       const int i = tri_row(q), mm = q - i * (i + 1) / 2;
       const int32_t x = ld<IDX32>(a.dis, tb + q);
-      M[i * FW_LD + mm] = x;
-      M[mm * FW_LD + i] = x;
+      M[i * LD + mm] = x;
+      M[mm * LD + i] = x;
     }
   }
-  for (int i = tid; i < FW_N; i += 256) M[i * FW_LD + i] = 0;
+  for (int i = tid; i < S; i += NT) M[i * LD + i] = 0;
   __syncthreads();
   {
     const int ib = c.tinner[ch.first];
     const int icnt = c.tinner[ch.first + n] - ib;
-    for (int q = tid; q < icnt; q += 256) {
+    for (int q = tid; q < icnt; q += NT) {
       const int i = c.inner_i[ib + q], k = c.inner_k[ib + q];
       const int32_t w = c.inner_w[ib + q];
-      M[i * FW_LD + k] = min(M[i * FW_LD + k], w);
-      if (SYM) M[k * FW_LD + i] = min(M[k * FW_LD + i], w);
+      M[i * LD + k] = min(M[i * LD + k], w);
+      if (SYM) M[k * LD + i] = min(M[k * LD + i], w);
     }
   }
   __syncthreads();
-  int32_t t[FW_RPT];
+  int32_t t[RPT];
 #pragma unroll
-  for (int r = 0; r < FW_RPT; r++) t[r] = M[(g * FW_RPT + r) * FW_LD + m];
-  for (int k = 0; k < n; k++) {
-    const int b = (k & 1) * FW_N;
-    if (g == (k >> 6)) {
-      int32_t v = t[0];
+  for (int r = 0; r < RPT; r++) t[r] = M[(g * RPT + r) * LD + m];
+  // step k = kg * RPT + kb + kk: row k lives in register kb + kk of group kg
+  for (int k0 = 0; k0 < n; k0 += KB) {
+    const int kg = k0 / RPT, kb = k0 % RPT;
 #pragma unroll
-      for (int r = 1; r < FW_RPT; r++) v = (r == (k & 63)) ? t[r] : v;
-      Rb[b + m] = v;
+    for (int kk = 0; kk < KB; kk++) {
+      const int k = k0 + kk;
+      if (k < n) {  // uniform
+        const int b = (k & 1) * S;
+        if (g == kg) {
+          int32_t v = t[kk];
+#pragma unroll
+          for (int r = KB; r < RPT; r += KB) v = (r == kb) ? t[r + kk] : v;
+          Rb[b + m] = v;
+        }
+        if (!SYM && m == k) {
+#pragma unroll
+          for (int r = 0; r < RPT; r++) Cb[b + g * RPT + r] = t[r];
+        }
+        __syncthreads();
+        const int32_t rkm = Rb[b + m];
+        const int32_t* col = SYM ? Rb + b + g * RPT : Cb + b + g * RPT;  // T[i][k] for this thread's rows
+#pragma unroll
+        for (int r = 0; r < RPT; r++) t[r] = min(t[r], __builtin_elementwise_add_sat(col[r], rkm));
+      }
     }
-    if (!SYM && m == k) {
-#pragma unroll
-      for (int r = 0; r < FW_RPT; r++) Cb[b + g * FW_RPT + r] = t[r];
-    }
-    __syncthreads();
-    const int32_t rkm = Rb[b + m];
-    const int32_t* col = SYM ? Rb + b + g * FW_RPT : Cb + b + g * FW_RPT;  // T[i][k] for this thread's rows
-#pragma unroll
-    for (int r = 0; r < FW_RPT; r++) t[r] = min(t[r], __builtin_elementwise_add_sat(col[r], rkm));
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < FW_RPT; r++) M[(g * FW_RPT + r) * FW_LD + m] = t[r];
+  for (int r = 0; r < RPT; r++) M[(g * RPT + r) * LD + m] = t[r];
   __syncthreads();
   if (MODE == FW_CLOSURE) {
     const uint64_t pb = (uint64_t)c.pro + ch.tri;
-    for (int q = tid; q < tot; q += 256) {
+    for (int q = tid; q < tot; q += NT) {
       const int i = tri_row(q);
-      a.dis[pb + q] = M[i * FW_LD + (q - i * (i + 1) / 2)];
+      a.dis[pb + q] = M[i * LD + (q - i * (i + 1) / 2)];
     }
     return;
   }
   const ChainRows row(c, ch, lane);
   for (int i = 0; i < n; i++) {
     const int64_t ri = row(i) + h;
-    for (int mm = tid; mm <= i; mm += 256) a.dis[ri + mm] = M[i * FW_LD + mm];
+    for (int mm = tid; mm <= i; mm += NT) a.dis[ri + mm] = M[i * LD + mm];
   }
   for (int mm = 0; mm < n; mm++) {
     const uint64_t tb = (uint64_t)a.tbo + (uint64_t)(h + mm) * (uint64_t)a.nt + ch.first;
-    for (int i = mm + tid; i < n; i += 256) a.dis[tb + i] = M[i * FW_LD + mm];
+    for (int i = mm + tid; i < n; i += NT) a.dis[tb + i] = M[i * LD + mm];
   }
 }
 
@@ -604,18 +633,27 @@ void set_lds_limit(K kernel) {
 
 size_t apply_lds(int lmax) { return (size_t)(lmax * 65 + lmax * (lmax + 1) / 2) * 4; }
 
+template <int MODE, int S>
+void launch_fw_s(const ChainArgs& c, int nunits, hipStream_t s) {
+  const dim3 block(fw_threads(S));
+  if (c.a.idx32) hipLaunchKernelGGL((chain_fw_kernel<MODE, S, true>), xcd_grid(nunits), block, fw_lds(S), s, c);
+  else hipLaunchKernelGGL((chain_fw_kernel<MODE, S, false>), xcd_grid(nunits), block, fw_lds(S), s, c);
+}
+
 template <int MODE>
-void launch_fw(const ChainArgs& c, int nunits, hipStream_t s) {
-  if (c.a.idx32) hipLaunchKernelGGL((chain_fw_kernel<MODE, true>), xcd_grid(nunits), dim3(256), fw_lds(), s, c);
-  else hipLaunchKernelGGL((chain_fw_kernel<MODE, false>), xcd_grid(nunits), dim3(256), fw_lds(), s, c);
+void launch_fw(const ChainArgs& c, int nunits, int lmax, hipStream_t s) {
+  const int S = fw_size(lmax);
+  if (S == 32) launch_fw_s<MODE, 32>(c, nunits, s);
+  else if (S == 64) launch_fw_s<MODE, 64>(c, nunits, s);
+  else launch_fw_s<MODE, 128>(c, nunits, s);
 }
 
 // The closure P of every chain (one launch, ahead of the rounds).
-void launch_chain_closure(const ChainArgs& base, int begin, int count, hipStream_t s) {
+void launch_chain_closure(const ChainArgs& base, int begin, int count, int lmax, hipStream_t s) {
   ChainArgs c = base;
   c.begin = begin;
   c.count = count;
-  launch_fw<FW_CLOSURE>(c, count, s);
+  launch_fw<FW_CLOSURE>(c, count, lmax, s);
 }
 
 // Step `step` (0 bound, 1 apply, 2 cross, 3 triangle) of one chain depth.
@@ -623,16 +661,7 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
   ChainArgs c = base;
   const bool l4 = c.a.l4 != 0, i32 = c.a.idx32 != 0;
   if (step == 0 && rd.k1n > 0) {
-    c.begin = rd.k1b;
-    c.count = rd.k1n;
-    const dim3 grid = xcd_grid((rd.k1n + 3) / 4);
-    if (i32) {
-      if (l4) hipLaunchKernelGGL((chain_bound_kernel<true, true>), grid, dim3(256), 0, s, c);
-      else hipLaunchKernelGGL((chain_bound_kernel<false, true>), grid, dim3(256), 0, s, c);
-    } else {
-      if (l4) hipLaunchKernelGGL((chain_bound_kernel<true, false>), grid, dim3(256), 0, s, c);
-      else hipLaunchKernelGGL((chain_bound_kernel<false, false>), grid, dim3(256), 0, s, c);
-    }
+    launch_bound(c.a, c.bunits, c.tpath, rd.k1b, rd.k1n, s);
   } else if (step == 1 && rd.k2n > 0) {
     c.begin = rd.k2b;
     c.count = rd.k2n;
@@ -648,20 +677,27 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
   } else if (step == 3 && rd.k4n > 0) {
     c.begin = rd.k4b;
     c.count = rd.k4n;
-    if (l4) launch_fw<FW_L4>(c, rd.k4n, s);
-    else launch_fw<FW_L3>(c, rd.k4n, s);
+    if (l4) launch_fw<FW_L4>(c, rd.k4n, rd.lmax, s);
+    else launch_fw<FW_L3>(c, rd.k4n, rd.lmax, s);
   }
+}
+
+template <int MODE>
+void set_fw_limits() {
+  set_lds_limit(chain_fw_kernel<MODE, 32, true>);
+  set_lds_limit(chain_fw_kernel<MODE, 32, false>);
+  set_lds_limit(chain_fw_kernel<MODE, 64, true>);
+  set_lds_limit(chain_fw_kernel<MODE, 64, false>);
+  set_lds_limit(chain_fw_kernel<MODE, 128, true>);
+  set_lds_limit(chain_fw_kernel<MODE, 128, false>);
 }
 
 void set_chain_lds_limits() {
   set_lds_limit(chain_apply_kernel<true>);
   set_lds_limit(chain_apply_kernel<false>);
-  set_lds_limit(chain_fw_kernel<FW_CLOSURE, true>);
-  set_lds_limit(chain_fw_kernel<FW_CLOSURE, false>);
-  set_lds_limit(chain_fw_kernel<FW_L3, true>);
-  set_lds_limit(chain_fw_kernel<FW_L3, false>);
-  set_lds_limit(chain_fw_kernel<FW_L4, true>);
-  set_lds_limit(chain_fw_kernel<FW_L4, false>);
+  set_fw_limits<FW_CLOSURE>();
+  set_fw_limits<FW_L3>();
+  set_fw_limits<FW_L4>();
 }
 // ---- subtree tasks (BATCHED bottom part) ----
 //
@@ -670,11 +706,11 @@ void set_chain_lds_limits() {
 // splits into
 //   outer columns j < r - 1 (ancestors above the task, all in chains):
 //     dis[v][j] = min(B_v(j), min over u in X(v) inside the task of
-//                     w + dis[u][j]),
-//     B_v(j) from the entries above the task, gathers from the transposed
-//     chain rows; one wavefront per (task, 64 columns) walks the task in DFS
-//     preorder, so the in-task ancestors' values of its columns are exactly
-//     a stack kept in LDS (task_outer_kernel);
+//                     w + dis[u][j]);
+//     bound_kernel writes B_v (entries above the task, gathers from the
+//     transposed chain rows), then one wavefront per (task, 64 columns) walks
+//     the task in DFS preorder: the in-task ancestors' values of its columns
+//     are exactly a stack kept in LDS (task_walk_kernel);
 //   inner columns r - 1 .. depth(v) - 2 (ancestors inside the task): one
 //     workgroup per task, level by level (task_inner_kernel).
 constexpr int TASK_H = 32;  // task height cap: the LDS stack depth
@@ -682,21 +718,15 @@ constexpr int WIN_N = 16;   // nodes per window of the DFS walk
 constexpr int WIN_E = 64;   // inner entries per window (one per lane)
 
 struct TaskInfo {
-  int32_t dfs;     // first node in the DFS-order arrays
-  int32_t nn;      // nodes
   int32_t win;     // first window
   int32_t nwin;    // windows
   int32_t r;       // depth of the root: the outer columns are [0, r - 1)
-  int32_t path;    // tpath offset of the chain holding the root's father
-  int32_t cfirst;  // chain index of that chain's head
-  int32_t ch0;     // depth of that chain's head
+  int32_t pad;
 };
 
 struct TaskWin {   // consecutive DFS nodes: at most WIN_N nodes and WIN_E inner entries
   int32_t k0, nn;  // first node (global DFS index), nodes
-  int32_t oe, no;  // outer entries
   int32_t ie, ni;  // inner entries
-  int32_t pad[2];
 };
 
 struct TaskArgs {
@@ -706,13 +736,9 @@ struct TaskArgs {
   const int64_t* dn_row;   // DFS nodes: row offset
   const int32_t* dn_tdep;  // task depth (the stack slot)
   const int32_t* dn_ie;    // first inner entry
-  const int32_t* oe_p;     // outer entries: position, weight,
-  const int32_t* oe_w;
-  const int64_t* oe_ro;    //   row offset of the bag member | node within the window << 56
   const uint8_t* ie_s;     // inner entries: task depth of the bag member, weight
   const int32_t* ie_w;
-  const int32_t* tpath;
-  const int4* units;       // task_outer_kernel: (task, first column)
+  const int4* units;       // task_walk_kernel: (task, first column)
   int nunits;
   const int32_t* lvl_base;   // [ntasks+1] first level of each task
   const int32_t* lvl_off;    // node offsets of every task level (BFS order)
@@ -729,23 +755,19 @@ __device__ inline int64_t readlane64(int64_t v, int l) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-template <bool L4, bool IDX32>
-__global__ void __launch_bounds__(256) task_outer_kernel(TaskArgs t) {
+template <bool IDX32>
+__global__ void __launch_bounds__(256) task_walk_kernel(TaskArgs t) {
   __shared__ int32_t lds_stack[4][TASK_H * 64];
-  __shared__ int32_t lds_b[4][WIN_N * 64];
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int u = xcd_index((t.nunits + 3) >> 2) * 4 + wid;
   if (u >= t.nunits) return;  // whole wavefronts; no workgroup barriers below
   int32_t* stack = lds_stack[wid];
-  int32_t* Bw = lds_b[wid];
   const LabelArgs& a = t.a;
   const int4 un = t.units[u];
   const TaskInfo T = t.tasks[un.x];
   const int L = T.r - 1;
   const int j = un.y + lane;
   const int jc = min(j, L - 1);  // clamped: every address stays valid
-  uint32_t ta = 0;               // chain index of a_j
-  if (L4) ta = jc < T.ch0 - 1 ? (uint32_t)t.tpath[T.path + jc] : (uint32_t)(T.cfirst + jc - (T.ch0 - 1));
   for (int wi = 0; wi < T.nwin; wi++) {
     const TaskWin W = t.wins[T.win + wi];
     // the window's node and inner-entry metadata, one per lane
@@ -760,57 +782,29 @@ __global__ void __launch_bounds__(256) task_outer_kernel(TaskArgs t) {
       is = t.ie_s[W.ie + lane];
       iw = t.ie_w[W.ie + lane];
     }
-    for (int q = 0; q < W.nn; q++) Bw[q * 64 + lane] = INF32;
-    // B of the window's nodes: outer entries, eight loads in flight
-    for (int e0 = 0; e0 < W.no; e0 += 64) {
-      int mp = 0, mw = 0;
-      int64_t mro = 0;
-      if (e0 + lane < W.no) {
-        mp = t.oe_p[W.oe + e0 + lane];
-        mw = t.oe_w[W.oe + e0 + lane];
-        mro = t.oe_ro[W.oe + e0 + lane];
-      }
-      const int kn = min(64, W.no - e0);
-      for (int q0 = 0; q0 < kn; q0 += 8) {
-        int32_t v[8], wv[8], qn[8];
-        bool use[8];
+    // the bounds of the window's nodes, all in flight at once
+    int32_t bv[WIN_N];
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-          const int e = min(q0 + r, kn - 1);
-          const int p = __builtin_amdgcn_readlane(mp, e);
-          wv[r] = __builtin_amdgcn_readlane(mw, e);
-          const int64_t ro_q = readlane64(mro, e);
-          qn[r] = (int)((uint64_t)ro_q >> 56);
-          const uint64_t ro = (uint64_t)ro_q & 0x00ffffffffffffffull;
-          use[r] = (q0 + r < kn) && (L4 || jc <= p);
-          const uint64_t idx = L4 ? ((jc <= p) ? ro + jc : (uint64_t)a.tbo + (uint64_t)(uint32_t)p * (uint64_t)a.nt + ta)
-                                  : ro + min(jc, p);
-          v[r] = ld<IDX32>(a.dis, idx);
-        }
+    for (int q = 0; q < WIN_N; q++) bv[q] = ld<IDX32>(a.dis, (uint64_t)readlane64(nrow, min(q, W.nn - 1)) + jc);
+    // walk: the in-task bag members of a node are on the stack
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
-          int32_t* b = &Bw[qn[r] * 64 + lane];
-          const int32_t old = *b;
-          *b = use[r] ? min(old, wv[r] + v[r]) : old;
-        }
+    for (int q = 0; q < WIN_N; q++) {
+      if (q < W.nn) {
+        int32_t x = bv[q];
+        const int i0 = __builtin_amdgcn_readlane(nie, q), i1 = __builtin_amdgcn_readlane(nie, q + 1);
+        for (int i = i0; i < i1; i++)
+          x = min(x, __builtin_amdgcn_readlane(iw, i) + stack[__builtin_amdgcn_readlane(is, i) * 64 + lane]);
+        stack[__builtin_amdgcn_readlane(ntd, q) * 64 + lane] = x;
+        const int64_t row = readlane64(nrow, q);
+        if (j < L) a.dis[row + j] = x;
       }
-    }
-    // walk the window's nodes: in-task bag members are on the stack
-    for (int q = 0; q < W.nn; q++) {
-      int32_t x = Bw[q * 64 + lane];
-      const int i0 = __builtin_amdgcn_readlane(nie, q), i1 = __builtin_amdgcn_readlane(nie, q + 1);
-      for (int i = i0; i < i1; i++)
-        x = min(x, __builtin_amdgcn_readlane(iw, i) + stack[__builtin_amdgcn_readlane(is, i) * 64 + lane]);
-      stack[__builtin_amdgcn_readlane(ntd, q) * 64 + lane] = x;
-      const int64_t row = readlane64(nrow, q);
-      if (j < L) a.dis[row + j] = x;
     }
   }
 }
 
 template <bool L4, bool IDX32>
 __global__ void __launch_bounds__(256) task_inner_kernel(TaskArgs t) {
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int task = xcd_index(t.ntasks);
   if (task >= t.ntasks) return;  // whole workgroup
   const LabelArgs& a = t.a;
@@ -865,13 +859,8 @@ void launch_tasks(const TaskArgs& t, hipStream_t s, int step) {
   const bool l4 = t.a.l4 != 0, i32 = t.a.idx32 != 0;
   if (step == 0) {
     const dim3 grid = xcd_grid((t.nunits + 3) / 4);
-    if (i32) {
-      if (l4) hipLaunchKernelGGL((task_outer_kernel<true, true>), grid, dim3(256), 0, s, t);
-      else hipLaunchKernelGGL((task_outer_kernel<false, true>), grid, dim3(256), 0, s, t);
-    } else {
-      if (l4) hipLaunchKernelGGL((task_outer_kernel<true, false>), grid, dim3(256), 0, s, t);
-      else hipLaunchKernelGGL((task_outer_kernel<false, false>), grid, dim3(256), 0, s, t);
-    }
+    if (i32) hipLaunchKernelGGL((task_walk_kernel<true>), grid, dim3(256), 0, s, t);
+    else hipLaunchKernelGGL((task_walk_kernel<false>), grid, dim3(256), 0, s, t);
   } else {
     const dim3 grid = xcd_grid(t.ntasks);
     if (i32) {
@@ -1062,19 +1051,31 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   }
   // work units of the four chain kernels, chain depth by chain depth
   std::vector<int4> units;
+  std::vector<BoundUnit> bunits;
   P.rounds.clear();
   for (size_t c0 = 0; c0 < corder.size();) {
     size_t c1 = c0;
     const int cd = hc[corder[c0]].cd;
     while (c1 < corder.size() && hc[corder[c1]].cd == cd) c1++;
     LabelPlan::Round rd{};
-    rd.k1b = (int)units.size();
+    rd.k1b = (int)bunits.size();
     for (size_t ci = c0; ci < c1; ci++) {
-      const int passes = std::max(1, (ch[ci].h0 - 1 + 64 * R - 1) / (64 * R));
+      const int L = ch[ci].h0 - 1;
       for (int i = 0; i < ch[ci].len; i++)
-        for (int q = 0; q < passes; q++) units.push_back(make_int4((int)ci, i, q * 64 * R, 0));
+        for (int q0 = 0; q0 < L; q0 += 64 * R) {
+          BoundUnit b;
+          b.v = tnode[ch[ci].first + i];
+          b.c0 = q0;
+          b.L = L;
+          b.nouter = touter[ch[ci].first + i];
+          b.path = ch[ci].path;
+          b.cfirst = ch[ci].first;
+          b.ch0 = ch[ci].h0;
+          b.pad = 0;
+          bunits.push_back(b);
+        }
     }
-    rd.k1n = (int)units.size() - rd.k1b;
+    rd.k1n = (int)bunits.size() - rd.k1b;
     rd.k2b = (int)units.size();
     for (size_t ci = c0; ci < c1; ci++)
       for (int j0 = 0; j0 < ch[ci].h0 - 1; j0 += 64) units.push_back(make_int4((int)ci, j0, 0, 0));
@@ -1096,6 +1097,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   P.kcl_b = (int)units.size();  // every chain, for the closure launch
   for (size_t ci = 0; ci < corder.size(); ci++) units.push_back(make_int4((int)ci, 0, 0, 0));
   P.kcl_n = (int)corder.size();
+  P.kcl_lmax = 0;
+  for (const auto& x : ch) P.kcl_lmax = std::max(P.kcl_lmax, (int)x.len);
   H2H_TRY(upload(ctx, P.chains, ch.data(), ch.size() * sizeof(Chain)));
   H2H_TRY(upload(ctx, P.tnode, tnode.data(), tnode.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.touter, touter.data(), touter.size() * sizeof(int32_t)));
@@ -1106,7 +1109,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   H2H_TRY(upload(ctx, P.tpath, tpath.data(), tpath.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.cunits, units.data(), units.size() * sizeof(int4)));
 
-  // tasks: BFS levels (inner columns), DFS streams and windows (outer columns)
+  // tasks: bound units, DFS walk windows (outer columns), BFS levels (inner columns)
   {
     std::vector<std::vector<int32_t>> members(ntasks);  // BFS order inside each task
     for (int i = 0; i < n; i++)
@@ -1115,19 +1118,35 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     std::vector<int64_t> tanc;
     std::vector<TaskInfo> infos(ntasks);
     std::vector<TaskWin> wins;
-    std::vector<int64_t> dn_row, oe_ro;
-    std::vector<int32_t> dn_tdep, dn_ie, oe_p, oe_w, ie_w;
+    std::vector<int64_t> dn_row;
+    std::vector<int32_t> dn_tdep, dn_ie, ie_w;
     std::vector<uint8_t> ie_s;
     std::vector<int4> tunits;
     std::vector<int32_t> local(n, -1);
+    P.tbound_b = (int)bunits.size();
     for (int t = 0; t < ntasks; t++) {
       const auto& mem = members[t];
       const int rt = mem[0];
       const int r = ctx->depth[rt];
+      const Chain& fc = ch[cpos[chain_of[ctx->father[rt]]]];
       for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
-      // BFS levels and in-task ancestor rows (phase B)
+      // bound units and BFS levels with the in-task ancestor rows
       for (size_t k = 0; k < mem.size(); k++) {
         const int v = mem[k];
+        int nouter = 0;
+        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1 && ctx->pos_pos[e] < r - 1; e++) nouter++;
+        for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
+          BoundUnit b;
+          b.v = v;
+          b.c0 = q0;
+          b.L = r - 1;
+          b.nouter = nouter;
+          b.path = fc.path;
+          b.cfirst = fc.first;
+          b.ch0 = fc.h0;
+          b.pad = 0;
+          bunits.push_back(b);
+        }
         if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
           starts.push_back((int32_t)bnodes.size());
           lvl_depth.push_back(ctx->depth[v]);
@@ -1144,7 +1163,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         }
       }
       lvl_base.push_back((int32_t)lvl_depth.size());
-      // DFS preorder
+      // DFS preorder and its windows
       std::vector<std::vector<int32_t>> kids(mem.size());
       for (size_t k = 1; k < mem.size(); k++) kids[local[ctx->father[mem[k]]]].push_back((int32_t)k);
       std::vector<int32_t> dfs, st(1, 0);
@@ -1156,61 +1175,46 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       }
       TaskInfo& I = infos[t];
       std::memset(&I, 0, sizeof(I));
-      I.dfs = (int32_t)dn_row.size();
-      I.nn = (int32_t)mem.size();
       I.win = (int32_t)wins.size();
       I.r = r;
-      {
-        const int f = ctx->father[rt];
-        const Chain& fc = ch[cpos[chain_of[f]]];
-        I.path = fc.path;
-        I.cfirst = fc.first;
-        I.ch0 = fc.h0;
-      }
       TaskWin W;
-      std::memset(&W, 0, sizeof(W));
-      auto close_win = [&]() {
-        if (W.nn > 0) wins.push_back(W);
-        std::memset(&W, 0, sizeof(W));
+      auto open_win = [&]() {
         W.k0 = (int32_t)dn_row.size();
-        W.oe = (int32_t)oe_p.size();
+        W.nn = 0;
         W.ie = (int32_t)ie_w.size();
+        W.ni = 0;
       };
-      close_win();
+      open_win();
       for (int k : dfs) {
         const int v = mem[k];
         const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1] - 1;
-        int no = 0, ni = 0;
-        for (int64_t e = e0; e < e1; e++) (ctx->pos_pos[e] < r - 1 ? no : ni)++;
-        if (W.nn == WIN_N || (W.nn > 0 && (W.ni + ni > WIN_E || W.no + no > 64))) close_win();
+        int ni = 0;
+        for (int64_t e = e0; e < e1; e++) ni += ctx->pos_pos[e] >= r - 1;
         if (ni > WIN_E) return set_err(ctx, H2H_ERR_INPUT, "task plan: node with more in-task bag members than a window holds");
-        const int q = W.nn;
+        if (W.nn == WIN_N || W.ni + ni > WIN_E) {
+          wins.push_back(W);
+          open_win();
+        }
         dn_row.push_back(ctx->dis_hash[v]);
         dn_tdep.push_back(ctx->depth[v] - r);
         dn_ie.push_back((int32_t)ie_w.size());
         for (int64_t e = e0; e < e1; e++) {
           const int pp = ctx->pos_pos[e];
-          if (pp < r - 1) {
-            oe_p.push_back(pp);
-            oe_w.push_back(ctx->pos_w[e]);
-            oe_ro.push_back(ctx->dis_hash[ctx->pos_id[e]] | ((int64_t)q << 56));
-          } else {
+          if (pp >= r - 1) {
             ie_s.push_back((uint8_t)(pp - (r - 1)));
             ie_w.push_back(ctx->pos_w[e]);
           }
         }
         W.nn++;
-        W.no += no;
         W.ni += ni;
       }
-      close_win();
+      if (W.nn > 0) wins.push_back(W);
       I.nwin = (int32_t)wins.size() - I.win;
       for (int j0 = 0; j0 < r - 1; j0 += 64) tunits.push_back(make_int4(t, j0, 0, 0));
       for (int v : mem) local[v] = -1;
     }
+    P.tbound_n = (int)bunits.size() - P.tbound_b;
     starts.push_back((int32_t)bnodes.size());
-    if (ctx->label_entries >= ((int64_t)1 << 56))
-      return set_err(ctx, H2H_ERR_OOM, "task plan: row offsets above 2^56");
     H2H_TRY(upload(ctx, P.task_lvl_base, lvl_base.data(), lvl_base.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_lvl_off, starts.data(), starts.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_lvl_depth, lvl_depth.data(), lvl_depth.size() * sizeof(int32_t)));
@@ -1222,9 +1226,6 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_dn_row, dn_row.data(), dn_row.size() * sizeof(int64_t)));
     H2H_TRY(upload(ctx, P.task_dn_tdep, dn_tdep.data(), dn_tdep.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_dn_ie, dn_ie.data(), dn_ie.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_oe_p, oe_p.data(), oe_p.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_oe_w, oe_w.data(), oe_w.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_oe_ro, oe_ro.data(), oe_ro.size() * sizeof(int64_t)));
     H2H_TRY(upload(ctx, P.task_ie_s, ie_s.data(), ie_s.size()));
     H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_units, tunits.data(), tunits.size() * sizeof(int4)));
@@ -1232,6 +1233,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     P.task_nodes_n = (int64_t)bnodes.size();
     P.task_nunits = (int)tunits.size();
   }
+  H2H_TRY(upload(ctx, P.bunits, bunits.data(), bunits.size() * sizeof(BoundUnit)));
   auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };
   P.nt = nt;
   P.tbo = up64(ctx->label_entries);
@@ -1291,10 +1293,11 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     c.inner_i = P.inner_i.as<uint8_t>();
     c.tpath = P.tpath.as<int32_t>();
     c.units = P.cunits.as<int4>();
+    c.bunits = P.bunits.as<BoundUnit>();
     c.tro = P.tro;
     c.pro = P.pro;
     c.begin = c.count = 0;
-    if (P.kcl_n > 0) H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b, P.kcl_n, s); }));
+    if (P.kcl_n > 0) H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b, P.kcl_n, P.kcl_lmax, s); }));
     for (const auto& rd : P.rounds) {
       const int n[4] = {rd.k1n, rd.k2n, a.l4 ? rd.k3n : 0, rd.k4n};
       for (int step = 0; step < 4; step++)
@@ -1309,12 +1312,8 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.dn_row = P.task_dn_row.as<int64_t>();
     t.dn_tdep = P.task_dn_tdep.as<int32_t>();
     t.dn_ie = P.task_dn_ie.as<int32_t>();
-    t.oe_p = P.task_oe_p.as<int32_t>();
-    t.oe_w = P.task_oe_w.as<int32_t>();
-    t.oe_ro = P.task_oe_ro.as<int64_t>();
     t.ie_s = P.task_ie_s.as<uint8_t>();
     t.ie_w = P.task_ie_w.as<int32_t>();
-    t.tpath = P.tpath.as<int32_t>();
     t.units = P.task_units.as<int4>();
     t.nunits = P.task_nunits;
     t.lvl_base = P.task_lvl_base.as<int32_t>();
@@ -1324,6 +1323,8 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.banc = P.task_banc.as<int32_t>();
     t.tanc = P.task_tanc.as<int64_t>();
     t.ntasks = P.ntasks;
+    if (P.tbound_n > 0)
+      H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), P.tbound_b, P.tbound_n, s); }));
     if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
     H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
   }
