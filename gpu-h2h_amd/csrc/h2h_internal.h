@@ -73,8 +73,8 @@ struct LabelPlan {
   DevArray items, init_nodes, init_depth;
   int64_t n_items = 0, n_init = 0;
   DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // subtree tasks: BFS levels
-  DevArray task_banc, task_tanc, task_info, task_wins, task_dn_row, task_dn_tdep, task_dn_ie;
-  DevArray task_ie_s, task_ie_w, task_units, bunits;  // DFS walk; bound units of chains and tasks
+  DevArray task_tbase, task_troot, task_banc, task_tanc, task_tanl, task_bn_ie, task_ie_s, task_ie_w;
+  DevArray task_cl_w, task_cl_ro, task_units, bunits;  // closure lists; bound units of chains and tasks
   int task_nunits = 0, tbound_b = 0, tbound_n = 0;
   int ntasks = 0;
   int64_t task_nodes_n = 0;

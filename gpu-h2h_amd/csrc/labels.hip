@@ -366,6 +366,12 @@ void launch_bound(const LabelArgs& a, const BoundUnit* units, const int32_t* tpa
   }
 }
 
+__device__ inline int64_t readlane64(int64_t v, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Row offsets of the chain's nodes, one per lane (node q in lane q of
 // r0 and lane q - 64 of r1), broadcast with readlane: no scalar load chain
 // per row.  Built with every lane active; the broadcast reads r0 and r1
@@ -701,104 +707,121 @@ void set_chain_lds_limits() {
 }
 // ---- subtree tasks (BATCHED bottom part) ----
 //
-// A task is a maximal subtree of at most smax nodes and TASK_H levels whose
-// root's father lies in a chain.  With r the depth of the root, a task row
-// splits into
+// A task is a maximal subtree of at most TASK_NODES nodes and TASK_H levels
+// whose root's father lies in a chain.  With r the depth of the root, a task
+// row splits into
 //   outer columns j < r - 1 (ancestors above the task, all in chains):
-//     dis[v][j] = min(B_v(j), min over u in X(v) inside the task of
-//                     w + dis[u][j]);
-//     bound_kernel writes B_v (entries above the task, gathers from the
-//     transposed chain rows), then one wavefront per (task, 64 columns) walks
-//     the task in DFS preorder: the in-task ancestors' values of its columns
-//     are exactly a stack kept in LDS (task_walk_kernel);
+//     dis[v][j] = min over u in A(v) of P[v][u] + B_u(j),
+//     A(v) = v and its in-task ancestors, B the bounds from the entries above
+//     the task (bound_kernel, gathers from the transposed chain rows) and P
+//     the closure of the in-task entries as a DAG (task_close_kernel, P[v][v]
+//     = 0).  task_apply_kernel reads each ancestor row, which holds B or the
+//     final value (either gives the same minimum), so no launch has to wait
+//     for another node of the same launch;
 //   inner columns r - 1 .. depth(v) - 2 (ancestors inside the task): one
 //     workgroup per task, level by level (task_inner_kernel).
-constexpr int TASK_H = 32;  // task height cap: the LDS stack depth
-constexpr int WIN_N = 16;   // nodes per window of the DFS walk
-constexpr int WIN_E = 64;   // inner entries per window (one per lane)
-
-struct TaskInfo {
-  int32_t win;     // first window
-  int32_t nwin;    // windows
-  int32_t r;       // depth of the root: the outer columns are [0, r - 1)
-  int32_t pad;
-};
-
-struct TaskWin {   // consecutive DFS nodes: at most WIN_N nodes and WIN_E inner entries
-  int32_t k0, nn;  // first node (global DFS index), nodes
-  int32_t ie, ni;  // inner entries
-};
+constexpr int TASK_H = 32;       // task height cap
+constexpr int TASK_NODES = 256;  // task size cap: the closure's LDS tile
 
 struct TaskArgs {
   LabelArgs a;
-  const TaskInfo* tasks;
-  const TaskWin* wins;
-  const int64_t* dn_row;   // DFS nodes: row offset
-  const int32_t* dn_tdep;  // task depth (the stack slot)
-  const int32_t* dn_ie;    // first inner entry
-  const uint8_t* ie_s;     // inner entries: task depth of the bag member, weight
-  const int32_t* ie_w;
-  const int4* units;       // task_walk_kernel: (task, first column)
-  int nunits;
   const int32_t* lvl_base;   // [ntasks+1] first level of each task
   const int32_t* lvl_off;    // node offsets of every task level (BFS order)
   const int32_t* lvl_depth;  // depth of the level's nodes
+  const int32_t* tbase;      // [ntasks] first BFS node of each task
+  const int32_t* troot;      // [ntasks] depth of the task root
   const int32_t* bnodes;     // vertices, task by task, level by level
-  const int32_t* banc;       // per BFS node: offset in tanc of its in-task ancestors' rows
-  const int64_t* tanc;
+  const int32_t* banc;       // per BFS node: offset of its in-task ancestors in tanc / tanl
+  const int64_t* tanc;       //   their row offsets (root first)
+  const uint16_t* tanl;      //   their index inside the task
+  const int32_t* bn_ie;      // per BFS node: first inner entry (BFS node count + 1 offsets)
+  const uint8_t* ie_s;       // inner entries: task depth of the bag member
+  const int32_t* ie_w;
+  int32_t* cl_w;             // closure weights: node k at banc[k] + k, d + 1 of them (self last)
+  const int64_t* cl_ro;      // their rows
+  const int4* units;         // task_apply_kernel: (BFS node, first column, outer columns, entries)
+  int nunits;
   int ntasks;
 };
 
-__device__ inline int64_t readlane64(int64_t v, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), l);
-  return (int64_t)(((uint64_t)hi << 32) | lo);
+// P[v][c] = min over the in-task entries (s, w) of v with s >= c of
+//   w + P[anc_s][c], P[x][depth(x)] = 0: one workgroup per task, level by
+// level, the task's P in LDS.
+__global__ void __launch_bounds__(256) task_close_kernel(TaskArgs t) {
+  __shared__ int32_t Pl[TASK_NODES * TASK_H];
+  const int task = xcd_index(t.ntasks);
+  if (task >= t.ntasks) return;  // whole workgroup
+  const int base = t.tbase[task], r = t.troot[task];
+  const int l1 = t.lvl_base[task + 1];
+  for (int li = t.lvl_base[task]; li < l1; li++) {
+    const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
+    const int d = t.lvl_depth[li] - r;
+    for (int q = threadIdx.x; q < (n1 - n0) * (d + 1); q += 256) {
+      const int k = n0 + q / (d + 1), c = q % (d + 1);
+      int32_t x = 0;
+      if (c < d) {
+        x = INF32;
+        const int e1 = t.bn_ie[k + 1];
+        for (int e = t.bn_ie[k]; e < e1; e++) {
+          const int s = t.ie_s[e];
+          if (s >= c) x = min(x, t.ie_w[e] + (s == c ? 0 : Pl[t.tanl[t.banc[k] + s] * TASK_H + c]));
+        }
+      }
+      Pl[(k - base) * TASK_H + c] = x;
+      t.cl_w[t.banc[k] + k + c] = x;
+    }
+    __syncthreads();
+  }
 }
 
+// Outer columns of task nodes: one wavefront per (node, 512 columns),
+// min over the closure entries (P, row) of w + dis[row][j], two entries per
+// round of loads.
 template <bool IDX32>
-__global__ void __launch_bounds__(256) task_walk_kernel(TaskArgs t) {
-  __shared__ int32_t lds_stack[4][TASK_H * 64];
+__global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int u = xcd_index((t.nunits + 3) >> 2) * 4 + wid;
-  if (u >= t.nunits) return;  // whole wavefronts; no workgroup barriers below
-  int32_t* stack = lds_stack[wid];
+  if (u >= t.nunits) return;  // whole wavefronts
   const LabelArgs& a = t.a;
   const int4 un = t.units[u];
-  const TaskInfo T = t.tasks[un.x];
-  const int L = T.r - 1;
-  const int j = un.y + lane;
-  const int jc = min(j, L - 1);  // clamped: every address stays valid
-  for (int wi = 0; wi < T.nwin; wi++) {
-    const TaskWin W = t.wins[T.win + wi];
-    // the window's node and inner-entry metadata, one per lane
-    int64_t nrow = 0;
-    int ntd = 0, nie = W.ni, is = 0, iw = 0;
-    if (lane < W.nn) {
-      nrow = t.dn_row[W.k0 + lane];
-      ntd = t.dn_tdep[W.k0 + lane];
-      nie = t.dn_ie[W.k0 + lane] - W.ie;
-    }
-    if (lane < W.ni) {
-      is = t.ie_s[W.ie + lane];
-      iw = t.ie_w[W.ie + lane];
-    }
-    // the bounds of the window's nodes, all in flight at once
-    int32_t bv[WIN_N];
+  const int k = un.x, c0 = un.y, L = un.z, ne = un.w;
+  const int64_t e0 = t.banc[k] + k;
+  const int64_t rowv = a.dis_hash[t.bnodes[k]];
+  int32_t acc[R];
 #pragma unroll
-    for (int q = 0; q < WIN_N; q++) bv[q] = ld<IDX32>(a.dis, (uint64_t)readlane64(nrow, min(q, W.nn - 1)) + jc);
-    // walk: the in-task bag members of a node are on the stack
+  for (int r = 0; r < R; r++) acc[r] = INF32;
+  int mw = 0;
+  int64_t mro = 0;
+  if (lane < ne) {  // at most TASK_H entries
+    mw = t.cl_w[e0 + lane];
+    mro = t.cl_ro[e0 + lane];
+  }
+  int q = 0;
+  for (; q + 1 < ne; q += 2) {
+    const int w0 = __builtin_amdgcn_readlane(mw, q), w1 = __builtin_amdgcn_readlane(mw, q + 1);
+    const uint64_t u0 = (uint64_t)readlane64(mro, q), u1 = (uint64_t)readlane64(mro, q + 1);
+    int32_t v0[R], v1[R];
 #pragma unroll
-    for (int q = 0; q < WIN_N; q++) {
-      if (q < W.nn) {
-        int32_t x = bv[q];
-        const int i0 = __builtin_amdgcn_readlane(nie, q), i1 = __builtin_amdgcn_readlane(nie, q + 1);
-        for (int i = i0; i < i1; i++)
-          x = min(x, __builtin_amdgcn_readlane(iw, i) + stack[__builtin_amdgcn_readlane(is, i) * 64 + lane]);
-        stack[__builtin_amdgcn_readlane(ntd, q) * 64 + lane] = x;
-        const int64_t row = readlane64(nrow, q);
-        if (j < L) a.dis[row + j] = x;
-      }
+    for (int r = 0; r < R; r++) {
+      const int j = min(c0 + r * 64 + lane, L - 1);
+      v0[r] = ld<IDX32>(a.dis, u0 + j);
+      v1[r] = ld<IDX32>(a.dis, u1 + j);
     }
+#pragma unroll
+    for (int r = 0; r < R; r++)  // L3 bounds can be INF: saturating adds
+      acc[r] = min(acc[r], min(__builtin_elementwise_add_sat(w0, v0[r]), __builtin_elementwise_add_sat(w1, v1[r])));
+  }
+  if (q < ne) {
+    const int w0 = __builtin_amdgcn_readlane(mw, q);
+    const uint64_t u0 = (uint64_t)readlane64(mro, q);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+      acc[r] = min(acc[r], __builtin_elementwise_add_sat(w0, ld<IDX32>(a.dis, u0 + min(c0 + r * 64 + lane, L - 1))));
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int j = c0 + r * 64 + lane;
+    if (j < L) a.dis[rowv + j] = acc[r];
   }
 }
 
@@ -808,7 +831,7 @@ __global__ void __launch_bounds__(256) task_inner_kernel(TaskArgs t) {
   const int task = xcd_index(t.ntasks);
   if (task >= t.ntasks) return;  // whole workgroup
   const LabelArgs& a = t.a;
-  const int r = t.tasks[task].r;
+  const int r = t.troot[task];
   const int l1 = t.lvl_base[task + 1];
   for (int li = t.lvl_base[task]; li < l1; li++) {
     const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
@@ -858,9 +881,11 @@ __global__ void __launch_bounds__(256) task_inner_kernel(TaskArgs t) {
 void launch_tasks(const TaskArgs& t, hipStream_t s, int step) {
   const bool l4 = t.a.l4 != 0, i32 = t.a.idx32 != 0;
   if (step == 0) {
+    hipLaunchKernelGGL(task_close_kernel, xcd_grid(t.ntasks), dim3(256), 0, s, t);
+  } else if (step == 1) {
     const dim3 grid = xcd_grid((t.nunits + 3) / 4);
-    if (i32) hipLaunchKernelGGL((task_walk_kernel<true>), grid, dim3(256), 0, s, t);
-    else hipLaunchKernelGGL((task_walk_kernel<false>), grid, dim3(256), 0, s, t);
+    if (i32) hipLaunchKernelGGL((task_apply_kernel<true>), grid, dim3(256), 0, s, t);
+    else hipLaunchKernelGGL((task_apply_kernel<false>), grid, dim3(256), 0, s, t);
   } else {
     const dim3 grid = xcd_grid(t.ntasks);
     if (i32) {
@@ -891,8 +916,9 @@ constexpr int CHAIN_MAX = 128;          // nodes per chain: the LDS tiles are si
 constexpr int CHAIN_INNER_MAX = 16384;  // bag entries inside one chain
 
 // Tuning knobs of the batched plan (environment, read when a plan is built):
-// H2H_TASK_NODES caps the nodes of a subtree task (default n / (2 * CUs), at
-// least 64), H2H_CHAIN_NODES the nodes of a chain (1..128, default 128).
+// H2H_TASK_NODES caps the nodes of a subtree task (1..256, default
+// n / (8 * CUs) within 64..256), H2H_CHAIN_NODES the nodes of a chain (1..128,
+// default 128).
 int64_t env_int(const char* name, int64_t dflt, int64_t lo, int64_t hi) {
   const char* v = std::getenv(name);
   if (!v || !*v) return dflt;
@@ -941,7 +967,9 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   const int n = ctx->n;
   // maximal subtrees of at most smax nodes and TASK_H levels become tasks
   std::vector<int32_t> task_of(n, -1);
-  const int64_t smax = env_int("H2H_TASK_NODES", std::max<int64_t>(64, n / (2LL * ctx->num_cus)), 1, INT32_MAX);
+  const int64_t smax = env_int("H2H_TASK_NODES",
+                               std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (8LL * ctx->num_cus))), 1,
+                               TASK_NODES);
   const int chain_max = (int)env_int("H2H_CHAIN_NODES", CHAIN_MAX, 1, CHAIN_MAX);
   int ntasks = 0;
   {
@@ -1109,19 +1137,17 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   H2H_TRY(upload(ctx, P.tpath, tpath.data(), tpath.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.cunits, units.data(), units.size() * sizeof(int4)));
 
-  // tasks: bound units, DFS walk windows (outer columns), BFS levels (inner columns)
+  // tasks: bound units, closure lists and apply units (outer columns), BFS
+  // levels (closure and inner columns)
   {
     std::vector<std::vector<int32_t>> members(ntasks);  // BFS order inside each task
     for (int i = 0; i < n; i++)
       if (task_of[ctx->bfs[i]] >= 0) members[task_of[ctx->bfs[i]]].push_back(ctx->bfs[i]);
-    std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth, bnodes, banc;
-    std::vector<int64_t> tanc;
-    std::vector<TaskInfo> infos(ntasks);
-    std::vector<TaskWin> wins;
-    std::vector<int64_t> dn_row;
-    std::vector<int32_t> dn_tdep, dn_ie, ie_w;
+    std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth, bnodes, banc, tbase, troot, bn_ie(1, 0), ie_w;
+    std::vector<int64_t> tanc, cl_ro;
+    std::vector<uint16_t> tanl;
     std::vector<uint8_t> ie_s;
-    std::vector<int4> tunits;
+    std::vector<int4> aunits;
     std::vector<int32_t> local(n, -1);
     P.tbound_b = (int)bunits.size();
     for (int t = 0; t < ntasks; t++) {
@@ -1129,10 +1155,13 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       const int rt = mem[0];
       const int r = ctx->depth[rt];
       const Chain& fc = ch[cpos[chain_of[ctx->father[rt]]]];
+      tbase.push_back((int32_t)bnodes.size());
+      troot.push_back(r);
       for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
-      // bound units and BFS levels with the in-task ancestor rows
       for (size_t k = 0; k < mem.size(); k++) {
         const int v = mem[k];
+        const int d = ctx->depth[v] - r;
+        const int kg = (int)bnodes.size();  // BFS node index
         int nouter = 0;
         for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1 && ctx->pos_pos[e] < r - 1; e++) nouter++;
         for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
@@ -1146,71 +1175,35 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           b.ch0 = fc.h0;
           b.pad = 0;
           bunits.push_back(b);
+          aunits.push_back(make_int4(kg, q0, r - 1, d + 1));
         }
         if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
-          starts.push_back((int32_t)bnodes.size());
+          starts.push_back(kg);
           lvl_depth.push_back(ctx->depth[v]);
         }
         bnodes.push_back(v);
         banc.push_back((int32_t)tanc.size());
-        const int d = ctx->depth[v] - r;
         const size_t at = tanc.size();
         tanc.resize(at + d);
+        tanl.resize(at + d);
         int x = v;
         for (int q = d - 1; q >= 0; q--) {
           x = ctx->father[x];
           tanc[at + q] = ctx->dis_hash[x];
+          tanl[at + q] = (uint16_t)local[x];
         }
-      }
-      lvl_base.push_back((int32_t)lvl_depth.size());
-      // DFS preorder and its windows
-      std::vector<std::vector<int32_t>> kids(mem.size());
-      for (size_t k = 1; k < mem.size(); k++) kids[local[ctx->father[mem[k]]]].push_back((int32_t)k);
-      std::vector<int32_t> dfs, st(1, 0);
-      while (!st.empty()) {
-        const int k = st.back();
-        st.pop_back();
-        dfs.push_back(k);
-        for (auto it = kids[k].rbegin(); it != kids[k].rend(); ++it) st.push_back(*it);
-      }
-      TaskInfo& I = infos[t];
-      std::memset(&I, 0, sizeof(I));
-      I.win = (int32_t)wins.size();
-      I.r = r;
-      TaskWin W;
-      auto open_win = [&]() {
-        W.k0 = (int32_t)dn_row.size();
-        W.nn = 0;
-        W.ie = (int32_t)ie_w.size();
-        W.ni = 0;
-      };
-      open_win();
-      for (int k : dfs) {
-        const int v = mem[k];
-        const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1] - 1;
-        int ni = 0;
-        for (int64_t e = e0; e < e1; e++) ni += ctx->pos_pos[e] >= r - 1;
-        if (ni > WIN_E) return set_err(ctx, H2H_ERR_INPUT, "task plan: node with more in-task bag members than a window holds");
-        if (W.nn == WIN_N || W.ni + ni > WIN_E) {
-          wins.push_back(W);
-          open_win();
-        }
-        dn_row.push_back(ctx->dis_hash[v]);
-        dn_tdep.push_back(ctx->depth[v] - r);
-        dn_ie.push_back((int32_t)ie_w.size());
-        for (int64_t e = e0; e < e1; e++) {
+        for (int q = 0; q < d; q++) cl_ro.push_back(tanc[at + q]);
+        cl_ro.push_back(ctx->dis_hash[v]);
+        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++) {
           const int pp = ctx->pos_pos[e];
           if (pp >= r - 1) {
             ie_s.push_back((uint8_t)(pp - (r - 1)));
             ie_w.push_back(ctx->pos_w[e]);
           }
         }
-        W.nn++;
-        W.ni += ni;
+        bn_ie.push_back((int32_t)ie_w.size());
       }
-      if (W.nn > 0) wins.push_back(W);
-      I.nwin = (int32_t)wins.size() - I.win;
-      for (int j0 = 0; j0 < r - 1; j0 += 64) tunits.push_back(make_int4(t, j0, 0, 0));
+      lvl_base.push_back((int32_t)lvl_depth.size());
       for (int v : mem) local[v] = -1;
     }
     P.tbound_n = (int)bunits.size() - P.tbound_b;
@@ -1219,19 +1212,20 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_lvl_off, starts.data(), starts.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_lvl_depth, lvl_depth.data(), lvl_depth.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_nodes, bnodes.data(), bnodes.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_tbase, tbase.data(), tbase.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_troot, troot.data(), troot.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_banc, banc.data(), banc.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_tanc, tanc.data(), tanc.size() * sizeof(int64_t)));
-    H2H_TRY(upload(ctx, P.task_info, infos.data(), infos.size() * sizeof(TaskInfo)));
-    H2H_TRY(upload(ctx, P.task_wins, wins.data(), wins.size() * sizeof(TaskWin)));
-    H2H_TRY(upload(ctx, P.task_dn_row, dn_row.data(), dn_row.size() * sizeof(int64_t)));
-    H2H_TRY(upload(ctx, P.task_dn_tdep, dn_tdep.data(), dn_tdep.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_dn_ie, dn_ie.data(), dn_ie.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_tanl, tanl.data(), tanl.size() * sizeof(uint16_t)));
+    H2H_TRY(upload(ctx, P.task_bn_ie, bn_ie.data(), bn_ie.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_ie_s, ie_s.data(), ie_s.size()));
     H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_units, tunits.data(), tunits.size() * sizeof(int4)));
+    H2H_TRY(upload(ctx, P.task_cl_ro, cl_ro.data(), cl_ro.size() * sizeof(int64_t)));
+    H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_ro.size(), 1) * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_units, aunits.data(), aunits.size() * sizeof(int4)));
     P.ntasks = ntasks;
     P.task_nodes_n = (int64_t)bnodes.size();
-    P.task_nunits = (int)tunits.size();
+    P.task_nunits = (int)aunits.size();
   }
   H2H_TRY(upload(ctx, P.bunits, bunits.data(), bunits.size() * sizeof(BoundUnit)));
   auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };
@@ -1307,26 +1301,28 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
   if (P.ntasks > 0) {
     TaskArgs t;
     t.a = a;
-    t.tasks = P.task_info.as<TaskInfo>();
-    t.wins = P.task_wins.as<TaskWin>();
-    t.dn_row = P.task_dn_row.as<int64_t>();
-    t.dn_tdep = P.task_dn_tdep.as<int32_t>();
-    t.dn_ie = P.task_dn_ie.as<int32_t>();
-    t.ie_s = P.task_ie_s.as<uint8_t>();
-    t.ie_w = P.task_ie_w.as<int32_t>();
-    t.units = P.task_units.as<int4>();
-    t.nunits = P.task_nunits;
     t.lvl_base = P.task_lvl_base.as<int32_t>();
     t.lvl_off = P.task_lvl_off.as<int32_t>();
     t.lvl_depth = P.task_lvl_depth.as<int32_t>();
+    t.tbase = P.task_tbase.as<int32_t>();
+    t.troot = P.task_troot.as<int32_t>();
     t.bnodes = P.task_nodes.as<int32_t>();
     t.banc = P.task_banc.as<int32_t>();
     t.tanc = P.task_tanc.as<int64_t>();
+    t.tanl = P.task_tanl.as<uint16_t>();
+    t.bn_ie = P.task_bn_ie.as<int32_t>();
+    t.ie_s = P.task_ie_s.as<uint8_t>();
+    t.ie_w = P.task_ie_w.as<int32_t>();
+    t.cl_w = P.task_cl_w.as<int32_t>();
+    t.cl_ro = P.task_cl_ro.as<int64_t>();
+    t.units = P.task_units.as<int4>();
+    t.nunits = P.task_nunits;
     t.ntasks = P.ntasks;
     if (P.tbound_n > 0)
       H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), P.tbound_b, P.tbound_n, s); }));
-    if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
-    H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+    H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
+    if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+    H2H_TRY(timed([&] { launch_tasks(t, s, 2); }));
   }
   return H2H_OK;
 }
