@@ -76,6 +76,7 @@ struct LabelPlan {
   DevArray task_tbase, task_troot, task_banc, task_tanc, task_tanl, task_bn_ie, task_ie_s, task_ie_w;
   DevArray task_cl_w, task_cl_ro, task_units, bunits;  // closure lists; bound units of chains and tasks
   int task_nunits = 0, tbound_b = 0, tbound_n = 0;
+  std::vector<int> tchunk_b, tchunk_a;  // task chunks: bound and apply unit offsets
   int ntasks = 0;
   int64_t task_nodes_n = 0;
   // BATCHED top part: the tree above the subtree tasks as vertical chains

@@ -330,12 +330,16 @@ struct BoundUnit {
   int32_t path, cfirst, ch0, pad;
 };
 
-template <bool L4, bool IDX32>
+template <bool L4, bool IDX32, bool SPLIT>
 __global__ void __launch_bounds__(256) bound_kernel(LabelArgs a, const BoundUnit* __restrict__ units,
                                                   const int32_t* __restrict__ tpath, int begin, int count) {
+  // SPLIT: the workgroup's four wavefronts share one unit's entries (stride 4)
+  // and combine through LDS -- for the wide bags of chain nodes; otherwise
+  // each wavefront takes its own unit
+  __shared__ int32_t comb[SPLIT ? 4 * 64 * R : 1];
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int u = xcd_index((count + 3) >> 2) * 4 + wid;
-  if (u >= count) return;  // whole wavefronts; no barriers below
+  const int u = SPLIT ? xcd_index(count) : xcd_index((count + 3) >> 2) * 4 + wid;
+  if (u >= count) return;  // SPLIT: whole workgroup; otherwise whole wavefronts
   const BoundUnit b = units[begin + u];
   const int64_t rowv = a.dis_hash[b.v];
   int32_t acc[R];
@@ -346,7 +350,25 @@ __global__ void __launch_bounds__(256) bound_kernel(LabelArgs a, const BoundUnit
     const int jj = min(b.c0 + r * 64 + lane, b.L - 1);
     g[r] = L4 ? (jj < b.ch0 - 1 ? (uint32_t)tpath[b.path + jj] : (uint32_t)(b.cfirst + jj - (b.ch0 - 1))) : 0u;
   }
-  accumulate<L4, IDX32, true>(a, a.pos_hash[b.v], 1, b.nouter, b.c0, b.L, lane, g, acc);
+  const int64_t e0 = a.pos_hash[b.v];
+  if (SPLIT) {
+    const int nk = b.nouter > wid ? (b.nouter - wid + 3) / 4 : 0;
+    accumulate<L4, IDX32, true>(a, e0 + wid, 4, nk, b.c0, b.L, lane, g, acc);
+#pragma unroll
+    for (int r = 0; r < R; r++) comb[(wid * R + r) * 64 + lane] = acc[r];
+    __syncthreads();
+    if (wid == 0) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int32_t x = min(min(comb[r * 64 + lane], comb[(R + r) * 64 + lane]),
+                              min(comb[(2 * R + r) * 64 + lane], comb[(3 * R + r) * 64 + lane]));
+        const int j = b.c0 + r * 64 + lane;
+        if (j < b.L) a.dis[rowv + j] = x;
+      }
+    }
+    return;
+  }
+  accumulate<L4, IDX32, true>(a, e0, 1, b.nouter, b.c0, b.L, lane, g, acc);
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int j = b.c0 + r * 64 + lane;
@@ -355,15 +377,24 @@ __global__ void __launch_bounds__(256) bound_kernel(LabelArgs a, const BoundUnit
 }
 
 void launch_bound(const LabelArgs& a, const BoundUnit* units, const int32_t* tpath, int begin, int count,
-                  hipStream_t s) {
-  const dim3 grid = xcd_grid((count + 3) / 4);
-  if (a.idx32) {
-    if (a.l4) hipLaunchKernelGGL((bound_kernel<true, true>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
-    else hipLaunchKernelGGL((bound_kernel<false, true>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
+                  hipStream_t s, bool split) {
+#define H2H_BOUND(L4_, I32_, SP_)                                                                   \
+  hipLaunchKernelGGL((bound_kernel<L4_, I32_, SP_>), grid, dim3(256), 0, s, a, units, tpath, begin, count)
+  const dim3 grid = split ? xcd_grid(count) : xcd_grid((count + 3) / 4);
+  if (split) {
+    if (a.idx32) {
+      if (a.l4) H2H_BOUND(true, true, true); else H2H_BOUND(false, true, true);
+    } else {
+      if (a.l4) H2H_BOUND(true, false, true); else H2H_BOUND(false, false, true);
+    }
   } else {
-    if (a.l4) hipLaunchKernelGGL((bound_kernel<true, false>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
-    else hipLaunchKernelGGL((bound_kernel<false, false>), grid, dim3(256), 0, s, a, units, tpath, begin, count);
+    if (a.idx32) {
+      if (a.l4) H2H_BOUND(true, true, false); else H2H_BOUND(false, true, false);
+    } else {
+      if (a.l4) H2H_BOUND(true, false, false); else H2H_BOUND(false, false, false);
+    }
   }
+#undef H2H_BOUND
 }
 
 __device__ inline int64_t readlane64(int64_t v, int l) {
@@ -462,9 +493,12 @@ __global__ void __launch_bounds__(256) chain_apply_kernel(ChainArgs c) {
 // diagonal 0 into row i of the chain's triangle scratch.
 template <bool IDX32>
 __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
+  // one workgroup per chain node; its four wavefronts split the entries
+  // (stride 4) and combine through LDS
+  __shared__ int32_t comb[4][128];
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int u = xcd_index((c.count + 3) >> 2) * 4 + wid;
-  if (u >= c.count) return;
+  const int u = xcd_index(c.count);
+  if (u >= c.count) return;  // whole workgroup
   const LabelArgs& a = c.a;
   const int4 un = c.units[c.begin + u];
   const Chain ch = c.chains[un.x];
@@ -476,7 +510,7 @@ __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
   const int m0 = min(lane, n - 1), m1 = min(lane + 64, n - 1);
   const bool two = i > 64;
   int32_t x0 = INF32, x1 = INF32;
-  for (int kb = 0; kb < nk; kb += 64) {
+  for (int kb = wid * 64; kb < nk; kb += 256) {
     int mp = 0, mw = 0;
     if (kb + lane < nk) {
       mp = a.pos_pos[e0 + kb + lane];
@@ -508,9 +542,16 @@ __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
       if (two) x1 = min(x1, w + ld<IDX32>(a.dis, rb + m1));
     }
   }
-  const uint64_t trow = (uint64_t)c.tro + ch.tri + (uint64_t)i * (i + 1) / 2;
-  if (lane <= i) a.dis[trow + lane] = lane == i ? 0 : x0;
-  if (lane + 64 <= i) a.dis[trow + lane + 64] = lane + 64 == i ? 0 : x1;
+  comb[wid][lane] = x0;
+  comb[wid][lane + 64] = x1;
+  __syncthreads();
+  if (wid == 0) {
+    x0 = min(min(comb[0][lane], comb[1][lane]), min(comb[2][lane], comb[3][lane]));
+    x1 = min(min(comb[0][lane + 64], comb[1][lane + 64]), min(comb[2][lane + 64], comb[3][lane + 64]));
+    const uint64_t trow = (uint64_t)c.tro + ch.tri + (uint64_t)i * (i + 1) / 2;
+    if (lane <= i) a.dis[trow + lane] = lane == i ? 0 : x0;
+    if (lane + 64 <= i) a.dis[trow + lane + 64] = lane + 64 == i ? 0 : x1;
+  }
 }
 
 // Floyd-Warshall over one chain's nodes in a workgroup of fw_threads(S)
@@ -667,7 +708,7 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
   ChainArgs c = base;
   const bool l4 = c.a.l4 != 0, i32 = c.a.idx32 != 0;
   if (step == 0 && rd.k1n > 0) {
-    launch_bound(c.a, c.bunits, c.tpath, rd.k1b, rd.k1n, s);
+    launch_bound(c.a, c.bunits, c.tpath, rd.k1b, rd.k1n, s, true);
   } else if (step == 1 && rd.k2n > 0) {
     c.begin = rd.k2b;
     c.count = rd.k2n;
@@ -677,7 +718,7 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
   } else if (step == 2 && rd.k3n > 0 && l4) {
     c.begin = rd.k3b;
     c.count = rd.k3n;
-    const dim3 grid = xcd_grid((rd.k3n + 3) / 4);
+    const dim3 grid = xcd_grid(rd.k3n);
     if (i32) hipLaunchKernelGGL((chain_cross_kernel<true>), grid, dim3(256), 0, s, c);
     else hipLaunchKernelGGL((chain_cross_kernel<false>), grid, dim3(256), 0, s, c);
   } else if (step == 3 && rd.k4n > 0) {
@@ -740,7 +781,7 @@ struct TaskArgs {
   int32_t* cl_w;             // closure weights: node k at banc[k] + k, d + 1 of them (self last)
   const int64_t* cl_ro;      // their rows
   const int4* units;         // task_apply_kernel: (BFS node, first column, outer columns, entries)
-  int nunits;
+  int ubegin, nunits;        // units of this launch
   int ntasks;
 };
 
@@ -783,7 +824,7 @@ __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   const int u = xcd_index((t.nunits + 3) >> 2) * 4 + wid;
   if (u >= t.nunits) return;  // whole wavefronts
   const LabelArgs& a = t.a;
-  const int4 un = t.units[u];
+  const int4 un = t.units[t.ubegin + u];
   const int k = un.x, c0 = un.y, L = un.z, ne = un.w;
   const int64_t e0 = t.banc[k] + k;
   const int64_t rowv = a.dis_hash[t.bnodes[k]];
@@ -1150,7 +1191,16 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     std::vector<int4> aunits;
     std::vector<int32_t> local(n, -1);
     P.tbound_b = (int)bunits.size();
+    // tasks run in chunks: a chunk's bounds are read back by the apply pass
+    // while they are still in the Infinity Cache
+    const int nchunks = (int)env_int("H2H_TASK_CHUNKS", 1, 1, 64);
+    P.tchunk_b.clear();
+    P.tchunk_a.clear();
     for (int t = 0; t < ntasks; t++) {
+      if ((int64_t)t * nchunks / std::max(ntasks, 1) != (int64_t)(t - 1) * nchunks / std::max(ntasks, 1) || t == 0) {
+        P.tchunk_b.push_back((int)bunits.size());
+        P.tchunk_a.push_back((int)aunits.size());
+      }
       const auto& mem = members[t];
       const int rt = mem[0];
       const int r = ctx->depth[rt];
@@ -1207,6 +1257,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       for (int v : mem) local[v] = -1;
     }
     P.tbound_n = (int)bunits.size() - P.tbound_b;
+    P.tchunk_b.push_back((int)bunits.size());
+    P.tchunk_a.push_back((int)aunits.size());
     starts.push_back((int32_t)bnodes.size());
     H2H_TRY(upload(ctx, P.task_lvl_base, lvl_base.data(), lvl_base.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_lvl_off, starts.data(), starts.size() * sizeof(int32_t)));
@@ -1316,12 +1368,18 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.cl_w = P.task_cl_w.as<int32_t>();
     t.cl_ro = P.task_cl_ro.as<int64_t>();
     t.units = P.task_units.as<int4>();
+    t.ubegin = 0;
     t.nunits = P.task_nunits;
     t.ntasks = P.ntasks;
-    if (P.tbound_n > 0)
-      H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), P.tbound_b, P.tbound_n, s); }));
     H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
-    if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+    for (size_t k = 0; k + 1 < P.tchunk_b.size(); k++) {
+      const int b0 = P.tchunk_b[k], b1 = P.tchunk_b[k + 1];
+      if (b1 > b0)
+        H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), b0, b1 - b0, s, false); }));
+      t.ubegin = P.tchunk_a[k];
+      t.nunits = P.tchunk_a[k + 1] - P.tchunk_a[k];
+      if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+    }
     H2H_TRY(timed([&] { launch_tasks(t, s, 2); }));
   }
   return H2H_OK;
