@@ -155,7 +155,8 @@ def main():
     if args.only_build:
         calibration_copy()
         kern_us, launches = p.build_labels_profiled(h2h.L4, h2h.BATCHED)
-        print(json.dumps({"label_build_ms": ms_per_step, "kernel_us": kern_us, "launches": launches}))
+        print(json.dumps({"workload": args.workload, "label_entries": entries, "label_build_ms": ms_per_step,
+                          "label_entries_per_s": value, "kernel_us": kern_us, "launches": launches}))
         return
     plain_build_us = [p.build_labels(h2h.L4, h2h.PF) for _ in range(3)]
     ll_build_us = [p.build_labels(h2h.L4, h2h.LL) for _ in range(3)]

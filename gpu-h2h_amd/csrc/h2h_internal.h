@@ -75,6 +75,8 @@ struct LabelPlan {
   DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // subtree tasks: BFS levels
   DevArray task_tbase, task_troot, task_banc, task_tanc, task_tanl, task_bn_ie, task_ie_s, task_ie_w;
   DevArray task_cl_w, task_cl_ro, task_units, bunits;  // closure lists; bound units of chains and tasks
+  DevArray task_tinfo;
+  bool task_levels = false;  // tasks by levels (one launch) or by closure (bound, close, apply, inner)
   int task_nunits = 0, tbound_b = 0, tbound_n = 0;
   std::vector<int> tchunk_b, tchunk_a;  // task chunks: bound and apply unit offsets
   int ntasks = 0;

@@ -284,6 +284,8 @@ __global__ void __launch_bounds__(256) label_init_kernel(LabelArgs a, const int3
 
 // ---- chains (BATCHED top part) ----
 
+constexpr int CHAIN_MAX = 128;  // nodes per chain: the LDS tiles are sized for it
+
 struct BoundUnit;
 
 struct Chain {
@@ -673,9 +675,11 @@ __global__ void __launch_bounds__(fw_threads(S)) chain_fw_kernel(ChainArgs c) {
   }
 }
 
+// Dynamic LDS above 64 KB has to be requested per kernel, for exactly what
+// its largest launch uses (static LDS counts against the same 160 KB).
 template <typename K>
-void set_lds_limit(K kernel) {
-  (void)hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+hipError_t set_lds_limit(K kernel, size_t bytes) {
+  return hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 size_t apply_lds(int lmax) { return (size_t)(lmax * 65 + lmax * (lmax + 1) / 2) * 4; }
@@ -730,21 +734,25 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
 }
 
 template <int MODE>
-void set_fw_limits() {
-  set_lds_limit(chain_fw_kernel<MODE, 32, true>);
-  set_lds_limit(chain_fw_kernel<MODE, 32, false>);
-  set_lds_limit(chain_fw_kernel<MODE, 64, true>);
-  set_lds_limit(chain_fw_kernel<MODE, 64, false>);
-  set_lds_limit(chain_fw_kernel<MODE, 128, true>);
-  set_lds_limit(chain_fw_kernel<MODE, 128, false>);
+hipError_t set_fw_limits() {
+  hipError_t e = hipSuccess;
+  auto chk = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+  chk(set_lds_limit(chain_fw_kernel<MODE, 32, true>, fw_lds(32)));
+  chk(set_lds_limit(chain_fw_kernel<MODE, 32, false>, fw_lds(32)));
+  chk(set_lds_limit(chain_fw_kernel<MODE, 64, true>, fw_lds(64)));
+  chk(set_lds_limit(chain_fw_kernel<MODE, 64, false>, fw_lds(64)));
+  chk(set_lds_limit(chain_fw_kernel<MODE, 128, true>, fw_lds(128)));
+  chk(set_lds_limit(chain_fw_kernel<MODE, 128, false>, fw_lds(128)));
+  return e;
 }
 
-void set_chain_lds_limits() {
-  set_lds_limit(chain_apply_kernel<true>);
-  set_lds_limit(chain_apply_kernel<false>);
-  set_fw_limits<FW_CLOSURE>();
-  set_fw_limits<FW_L3>();
-  set_fw_limits<FW_L4>();
+hipError_t set_chain_lds_limits() {
+  hipError_t e = set_lds_limit(chain_apply_kernel<true>, apply_lds(CHAIN_MAX));
+  if (e == hipSuccess) e = set_lds_limit(chain_apply_kernel<false>, apply_lds(CHAIN_MAX));
+  if (e == hipSuccess) e = set_fw_limits<FW_CLOSURE>();
+  if (e == hipSuccess) e = set_fw_limits<FW_L3>();
+  if (e == hipSuccess) e = set_fw_limits<FW_L4>();
+  return e;
 }
 // ---- subtree tasks (BATCHED bottom part) ----
 //
@@ -771,6 +779,7 @@ struct TaskArgs {
   const int32_t* lvl_depth;  // depth of the level's nodes
   const int32_t* tbase;      // [ntasks] first BFS node of each task
   const int32_t* troot;      // [ntasks] depth of the task root
+  const int4* tinfo;         // [ntasks] (root depth, tpath offset, head index, head depth) of the chain above
   const int32_t* bnodes;     // vertices, task by task, level by level
   const int32_t* banc;       // per BFS node: offset of its in-task ancestors in tanc / tanl
   const int64_t* tanc;       //   their row offsets (root first)
@@ -919,6 +928,97 @@ __global__ void __launch_bounds__(256) task_inner_kernel(TaskArgs t) {
   }
 }
 
+// One workgroup per task, level by level: every row of a level from its
+// whole bag -- the outer columns (ancestors above the task; gathers from the
+// transposed chain rows) in 512-column units, the inner columns in one unit
+// (gathers from the in-task ancestors' rows).  In-task bag members sit on
+// earlier levels, so their rows are final.
+template <bool L4, bool IDX32>
+__global__ void __launch_bounds__(256) task_level_kernel(TaskArgs t, const int32_t* __restrict__ tpath) {
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int task = xcd_index(t.ntasks);
+  if (task >= t.ntasks) return;  // whole workgroup
+  const LabelArgs& a = t.a;
+  const int4 ti = t.tinfo[task];
+  const int r = ti.x, L = r - 1;
+  const int passes = (L + 64 * R - 1) / (64 * R);
+  const int l1 = t.lvl_base[task + 1];
+  for (int li = t.lvl_base[task]; li < l1; li++) {
+    const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
+    const int d = t.lvl_depth[li] - r;
+    const int units = (n1 - n0) * (passes + 1);
+    for (int uu = wid; uu < units; uu += 4) {
+      const int k = n0 + uu / (passes + 1), q = uu % (passes + 1);
+      const int v = t.bnodes[k];
+      const int64_t rowv = a.dis_hash[v];
+      const int64_t e0 = a.pos_hash[v];
+      const int nk = (int)(a.pos_hash[v + 1] - 1 - e0);
+      if (q < passes) {
+        const int c0 = q * 64 * R;
+        int32_t acc[R];
+        uint32_t g[R];
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) {
+          acc[rr] = INF32;
+          const int jj = min(c0 + rr * 64 + lane, L - 1);
+          g[rr] = L4 ? (jj < ti.w - 1 ? (uint32_t)tpath[ti.y + jj] : (uint32_t)(ti.z + jj - (ti.w - 1))) : 0u;
+        }
+        accumulate<L4, IDX32, true>(a, e0, 1, nk, c0, L, lane, g, acc);
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) {
+          const int j = c0 + rr * 64 + lane;
+          if (j < L) a.dis[rowv + j] = acc[rr];
+        }
+      } else {
+        if (d > 0) {
+          const int j = r - 1 + min(lane, d - 1);
+          const int64_t arow = t.tanc[t.banc[k] + min(lane, d - 1)];  // a_j: in-task ancestor at task depth lane
+          int32_t x = INF32;
+          for (int kb = 0; kb < nk; kb += 64) {
+            int mp = 0, mw = 0;
+            int64_t mro = 0;
+            if (kb + lane < nk) {
+              mp = a.pos_pos[e0 + kb + lane];
+              mw = a.pos_w[e0 + kb + lane];
+              mro = a.pos_rowoff[e0 + kb + lane];
+            }
+            const int kn = min(64, nk - kb);
+            for (int q0 = 0; q0 < kn; q0 += 4) {
+              int32_t y[4], wv[4];
+              bool use[4];
+#pragma unroll
+              for (int qq = 0; qq < 4; qq++) {
+                const int e = min(q0 + qq, kn - 1);
+                const int p = __builtin_amdgcn_readlane(mp, e);
+                wv[qq] = __builtin_amdgcn_readlane(mw, e);
+                const int64_t ro = readlane64(mro, e);
+                use[qq] = (q0 + qq < kn) && (L4 || j <= p);
+                y[qq] = ld<IDX32>(a.dis, L4 ? ((j <= p) ? (uint64_t)ro + j : (uint64_t)arow + p) : (uint64_t)ro + min(j, p));
+              }
+#pragma unroll
+              for (int qq = 0; qq < 4; qq++) x = use[qq] ? min(x, wv[qq] + y[qq]) : x;
+            }
+          }
+          if (lane < d) a.dis[rowv + r - 1 + lane] = x;
+        }
+        if (lane == 0) a.dis[rowv + r - 1 + d] = 0;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+void launch_task_levels(const TaskArgs& t, const int32_t* tpath, hipStream_t s) {
+  const dim3 grid = xcd_grid(t.ntasks);
+  if (t.a.idx32) {
+    if (t.a.l4) hipLaunchKernelGGL((task_level_kernel<true, true>), grid, dim3(256), 0, s, t, tpath);
+    else hipLaunchKernelGGL((task_level_kernel<false, true>), grid, dim3(256), 0, s, t, tpath);
+  } else {
+    if (t.a.l4) hipLaunchKernelGGL((task_level_kernel<true, false>), grid, dim3(256), 0, s, t, tpath);
+    else hipLaunchKernelGGL((task_level_kernel<false, false>), grid, dim3(256), 0, s, t, tpath);
+  }
+}
+
 void launch_tasks(const TaskArgs& t, hipStream_t s, int step) {
   const bool l4 = t.a.l4 != 0, i32 = t.a.idx32 != 0;
   if (step == 0) {
@@ -953,13 +1053,14 @@ namespace {
 //   PF       one item per node (its whole bag), 256 threads, one launch per level.
 //   LL       one item per (node, bag entry), 64 threads, atomic combine.
 //   BATCHED  subtree tasks below, chains above (see the top of this file).
-constexpr int CHAIN_MAX = 128;          // nodes per chain: the LDS tiles are sized for it
 constexpr int CHAIN_INNER_MAX = 16384;  // bag entries inside one chain
 
 // Tuning knobs of the batched plan (environment, read when a plan is built):
 // H2H_TASK_NODES caps the nodes of a subtree task (1..256, default
 // n / (8 * CUs) within 64..256), H2H_CHAIN_NODES the nodes of a chain (1..128,
-// default 128).
+// default 128), H2H_TASK_LEVELS=1 builds the tasks level by level in one
+// launch instead of by closure (bound, close, apply, inner), H2H_TASK_CHUNKS
+// splits the tasks' bound and apply passes into chunks (default 1).
 int64_t env_int(const char* name, int64_t dflt, int64_t lo, int64_t hi) {
   const char* v = std::getenv(name);
   if (!v || !*v) return dflt;
@@ -1188,7 +1289,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     std::vector<int64_t> tanc, cl_ro;
     std::vector<uint16_t> tanl;
     std::vector<uint8_t> ie_s;
-    std::vector<int4> aunits;
+    std::vector<int4> aunits, tinfo;
     std::vector<int32_t> local(n, -1);
     P.tbound_b = (int)bunits.size();
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
@@ -1207,7 +1308,10 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       const Chain& fc = ch[cpos[chain_of[ctx->father[rt]]]];
       tbase.push_back((int32_t)bnodes.size());
       troot.push_back(r);
+      tinfo.push_back(make_int4(r, fc.path, fc.first, fc.h0));
       for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
+      std::vector<BoundUnit> tb_units;
+      std::vector<int4> ta_units;
       for (size_t k = 0; k < mem.size(); k++) {
         const int v = mem[k];
         const int d = ctx->depth[v] - r;
@@ -1224,8 +1328,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           b.cfirst = fc.first;
           b.ch0 = fc.h0;
           b.pad = 0;
-          bunits.push_back(b);
-          aunits.push_back(make_int4(kg, q0, r - 1, d + 1));
+          tb_units.push_back(b);
+          ta_units.push_back(make_int4(kg, q0, r - 1, d + 1));
         }
         if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
           starts.push_back(kg);
@@ -1254,6 +1358,15 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         bn_ie.push_back((int32_t)ie_w.size());
       }
       lvl_base.push_back((int32_t)lvl_depth.size());
+      // the task's units pass by pass: neighbouring units (one workgroup's
+      // wavefronts) are sibling nodes on the same columns, sharing the
+      // ancestor rows they read through the CU's L1
+      for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
+        for (const auto& b : tb_units)
+          if (b.c0 == q0) bunits.push_back(b);
+        for (const auto& x : ta_units)
+          if (x.y == q0) aunits.push_back(x);
+      }
       for (int v : mem) local[v] = -1;
     }
     P.tbound_n = (int)bunits.size() - P.tbound_b;
@@ -1266,6 +1379,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_nodes, bnodes.data(), bnodes.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_tbase, tbase.data(), tbase.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_troot, troot.data(), troot.size() * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_tinfo, tinfo.data(), tinfo.size() * sizeof(int4)));
+    P.task_levels = env_int("H2H_TASK_LEVELS", 0, 0, 1) != 0;
     H2H_TRY(upload(ctx, P.task_banc, banc.data(), banc.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_tanc, tanc.data(), tanc.size() * sizeof(int64_t)));
     H2H_TRY(upload(ctx, P.task_tanl, tanl.data(), tanl.size() * sizeof(uint16_t)));
@@ -1371,6 +1486,11 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.ubegin = 0;
     t.nunits = P.task_nunits;
     t.ntasks = P.ntasks;
+    t.tinfo = P.task_tinfo.as<int4>();
+    if (P.task_levels) {
+      H2H_TRY(timed([&] { launch_task_levels(t, P.tpath.as<int32_t>(), s); }));
+      return H2H_OK;
+    }
     H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
     for (size_t k = 0; k + 1 < P.tchunk_b.size(); k++) {
       const int b0 = P.tchunk_b[k], b1 = P.tchunk_b[k + 1];
@@ -1395,7 +1515,7 @@ h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, Label
   const bool anc = l4 && method != H2H_METHOD_BATCHED;  // ancestor rows: the level kernels only
   if (anc) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
   H2H_TRY(dev_alloc(ctx, ctx->d_dis, std::max<int64_t>(P.dis_entries, 1) * sizeof(int32_t)));
-  if (method == H2H_METHOD_BATCHED) set_chain_lds_limits();
+  if (method == H2H_METHOD_BATCHED) H2H_HIP(ctx, set_chain_lds_limits());
   a.bfs = ctx->d_bfs.as<int32_t>();
   a.father = ctx->d_father.as<int32_t>();
   a.dis_hash = ctx->d_dis_hash.as<int64_t>();

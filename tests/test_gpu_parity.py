@@ -153,13 +153,18 @@ def test_labels_upload_roundtrip():
             np.testing.assert_array_equal(p.labels()["dis"], o.dis())
 
 
-@pytest.mark.parametrize("task_nodes,chain_nodes", [(1, 1), (16, 3), (64, 128), (300, 7), (10 ** 9, 128)])
-def test_batched_plan_shapes(monkeypatch, task_nodes, chain_nodes):
+@pytest.mark.parametrize("task_nodes,chain_nodes,levels,chunks", [(1, 1, 0, 1), (16, 3, 1, 3), (64, 128, 0, 4),
+                                                                  (64, 128, 1, 1), (300, 7, 0, 2),
+                                                                  (10 ** 9, 128, 1, 1)])
+def test_batched_plan_shapes(monkeypatch, task_nodes, chain_nodes, levels, chunks):
     """The batched build at every split between subtree tasks and chains:
     chains of one node (one chain depth per tree level), short chains that
-    split separators, the default caps, and a tree that is all tasks."""
+    split separators, the default caps, and the largest tasks; tasks by
+    closure (in one or several chunks) and level by level."""
     monkeypatch.setenv("H2H_TASK_NODES", str(task_nodes))
     monkeypatch.setenv("H2H_CHAIN_NODES", str(chain_nodes))
+    monkeypatch.setenv("H2H_TASK_LEVELS", str(levels))
+    monkeypatch.setenv("H2H_TASK_CHUNKS", str(chunks))
     for case, C in ((cases.grid64(), 1), (cases.delaunay(30000, 2.6, 5, 7), 2)):
         o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=4)
         with p:
@@ -168,3 +173,29 @@ def test_batched_plan_shapes(monkeypatch, task_nodes, chain_nodes):
                 for _ in range(2):   # graph replays
                     p.build_labels(kind, h2h.BATCHED)
                     np.testing.assert_array_equal(p.labels()["dis"], o.dis())
+
+
+def test_fla1m_batched_matches_plain_and_dijkstra():
+    """The 1.07M-vertex FLA-scale workload: the batched L4 build equals the
+    per-node build entry for entry, and sampled rows of both query kinds
+    equal Dijkstra."""
+    import h2h_gen as G
+    g, ids, cut, H = G.make_workload("fla1m")
+    u, v, w = g.arcs()
+    with h2h.H2HIndex() as p:
+        p.set_graph(g.n, u, v, w, ids, cut, H)
+        p.contract(1, h2h.CONTRACT_GPU_ORDER, 8)
+        p.build_labels(h2h.L4, h2h.BATCHED)
+        batched = p.labels()["dis"]
+        p.build_labels(h2h.L4, h2h.PF)
+        np.testing.assert_array_equal(p.labels()["dis"], batched)
+        del batched
+        p.build_labels(h2h.L4, h2h.BATCHED)
+        case = cases.Case("fla1m", g.n, u, v, w, ids, cut, H)
+        src = np.random.default_rng(5).integers(0, g.n, 6)
+        D = case.dijkstra_rows(src)
+        allv = np.arange(g.n, dtype=np.int32)
+        for i, s in enumerate(src):
+            for kind in (h2h.QUERY_LCA, h2h.QUERY_NOLCA):
+                got, _ = p.query(np.full(g.n, s, np.int32), allv, kind, h2h.EXEC_BATCHED)
+                np.testing.assert_array_equal(got, D[i])
