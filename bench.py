@@ -116,6 +116,100 @@ def served_latency(p, n, rate_qps, batches, size, seed):
     return np.array(lat)
 
 
+def serving_sweep(p, n, peak_qps, requests=400, seed=11):
+    """Poisson request arrivals at 10/30/50/70/90% of the plain kernel's peak
+    query rate, request sizes log-uniform in 1..65536, into the serving loop
+    running the plain and the batched kernel; per-request latency percentiles
+    and the achieved rate, with answers checked against direct queries."""
+    rng = np.random.default_rng(seed)
+    sizes = np.minimum(65536, np.exp(rng.uniform(0.0, np.log(65536.0), requests))).astype(np.int64)
+    xs = [rng.integers(0, n, k).astype(np.int32) for k in sizes]
+    ys = [rng.integers(0, n, k).astype(np.int32) for k in sizes]
+    check = [int(i) for i in rng.choice(requests, 4, replace=False)]
+    ref = {i: p.query(xs[i], ys[i], h2h.QUERY_LCA, h2h.EXEC_BATCHED)[0] for i in check}
+    out = {}
+    for ename, ex in (("plain", h2h.EXEC_PLAIN), ("batched", h2h.EXEC_BATCHED)):
+        pts = []
+        for frac in (0.1, 0.3, 0.5, 0.7, 0.9):
+            rate = frac * peak_qps
+            gaps = rng.exponential(sizes.mean() / rate, requests)
+            p.server_start(streams=4, max_inflight=1 << 20, max_request=65536, kind=h2h.QUERY_LCA, exec_mode=ex)
+            tickets, outs, stats = [], {}, []
+            try:
+                def waiter():
+                    i = 0
+                    while i < requests:
+                        if i >= len(tickets):
+                            time.sleep(0.00005)
+                            continue
+                        stats.append(p.wait(tickets[i]))
+                        i += 1
+
+                th = threading.Thread(target=waiter)
+                th.start()
+                t_next = time.perf_counter()
+                for k in range(requests):
+                    t_next += gaps[k]
+                    while time.perf_counter() < t_next:
+                        pass
+                    t, o = p.submit(xs[k], ys[k])
+                    if k in ref:
+                        outs[k] = o
+                    tickets.append(t)
+                th.join()
+            finally:
+                p.server_stop()
+            for k, o in outs.items():
+                if not np.array_equal(o, ref[k]):
+                    raise SystemExit(f"served answers differ from direct queries ({ename}, request {k})")
+            lat = np.array([(st["done_us"] - st["submit_us"]) / 1000.0 for st in stats])
+            span = (max(st["done_us"] for st in stats) - min(st["submit_us"] for st in stats)) * 1e-6
+            pts.append({"offered_frac": frac, "offered_qps": rate, "achieved_qps": float(sizes.sum() / span),
+                        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))})
+        out[ename] = pts
+    return out
+
+
+def scale_run(cores, steps):
+    """The FLA-scale configuration (1.07M vertices, H=12): batched L4 build
+    against the CPU oracle's, query rates, and the serving sweep."""
+    g, ids, cut, H = G.make_workload("fla1m")
+    u, v, w = g.arcs()
+    res = {"workload": f"fla1m: {g.n} vertices, {g.m} edges, coordinate bisection H={H}, ChangeToGlobalHeight=1"}
+    with h2h.H2HIndex(0) as p:
+        p.set_graph(g.n, u, v, w, ids, cut, H)
+        res["contraction_device_s"] = p.contract(1, h2h.CONTRACT_GPU_ORDER, cores) * 1e-6
+        entries = p.info()["label_entries"]
+        p.build_labels(h2h.L4, h2h.BATCHED)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            p.build_labels(h2h.L4, h2h.BATCHED)
+        build_s = (time.perf_counter() - t0) / steps
+        res.update({"label_entries": entries, "label_build_s": build_s, "label_entries_per_s": entries / build_s})
+        rng = np.random.default_rng(21)
+        x = rng.integers(0, g.n, 1000000).astype(np.int32)
+        y = rng.integers(0, g.n, 1000000).astype(np.int32)
+        q = {}
+        for ex, ename in ((h2h.EXEC_PLAIN, "plain"), (h2h.EXEC_BATCHED, "batched")):
+            q[f"lca_{ename}_qps"] = 1e6 / (min(p.query(x, y, h2h.QUERY_LCA, ex)[1] for _ in range(5)) * 1e-6)
+        res["queries"] = q
+        res["serving"] = serving_sweep(p, g.n, q["lca_plain_qps"])
+        gpu_ans = p.query(x[:200000], y[:200000], h2h.QUERY_NOLCA)[0]
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    o = O.OracleIndex(g.n, u, v, w, ids, cut, H, threads=cores)
+    o.contract(O.GPU_FAST, 1)                       # untimed preparation
+    cpu_us = o.labels(O.PF, l4=True, threads=cores)
+    cq, _ = o.query(O.Q_NOLCA, x[:200000], y[:200000], threads=cores)
+    if not np.array_equal(cq, gpu_ans):
+        raise SystemExit("CPU oracle and GPU answers differ at FLA scale")
+    res["cpu_label_build_s"] = cpu_us * 1e-6
+    res["cpu_threads"] = cores
+    res["speedup_vs_cpu_label_build"] = cpu_us * 1e-6 / build_s
+    o.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +217,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="ny264k")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-scale", action="store_true", help="skip the FLA-scale (1.07M vertices) section")
     ap.add_argument("--only-build", action="store_true",
                     help="label build steps and the profiled pass only (counter runs)")
     args = ap.parse_args()
@@ -279,6 +374,9 @@ def main():
     }
     if cpu:
         out["speedup_vs_cpu_label_build"] = cpu["label_build_s"] / (ms_per_step / 1000.0)
+    if not args.no_scale and args.workload == "ny264k":
+        del p
+        out["fla1m"] = scale_run(cpu_cores(), 5)
     print(json.dumps(out))
 
 
