@@ -600,14 +600,24 @@ __global__ void __launch_bounds__(fw_threads(S)) chain_fw_kernel(ChainArgs c) {
   int32_t* Cb = Rb + 2 * S;    // [2][S] column k
   for (int q = tid; q < S * LD; q += NT) M[q] = INF32;
   __syncthreads();
+  // set-up loads are issued in batches of eight before their LDS stores:
+  // one memory round trip per batch instead of per element
   if (SYM) {
     // t_i(m) edges and the diagonal from the cross scratch
     const uint64_t tb = (uint64_t)c.tro + ch.tri;
-    for (int q = tid; q < tot; q += NT) {
-      const int i = tri_row(q), mm = q - i * (i + 1) / 2;
-      const int32_t x = ld<IDX32>(a.dis, tb + q);
-      M[i * LD + mm] = x;
-      M[mm * LD + i] = x;
+    for (int q0 = 0; q0 < tot; q0 += 8 * NT) {
+      int32_t x[8];
+#pragma unroll
+      for (int r = 0; r < 8; r++) x[r] = ld<IDX32>(a.dis, tb + min(q0 + r * NT + tid, tot - 1));
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int q = q0 + r * NT + tid;
+        if (q < tot) {
+          const int i = tri_row(q), mm = q - i * (i + 1) / 2;
+          M[i * LD + mm] = x[r];
+          M[mm * LD + i] = x[r];
+        }
+      }
     }
   }
   for (int i = tid; i < S; i += NT) M[i * LD + i] = 0;
@@ -615,11 +625,23 @@ __global__ void __launch_bounds__(fw_threads(S)) chain_fw_kernel(ChainArgs c) {
   {
     const int ib = c.tinner[ch.first];
     const int icnt = c.tinner[ch.first + n] - ib;
-    for (int q = tid; q < icnt; q += NT) {
-      const int i = c.inner_i[ib + q], k = c.inner_k[ib + q];
-      const int32_t w = c.inner_w[ib + q];
-      M[i * LD + k] = min(M[i * LD + k], w);
-      if (SYM) M[k * LD + i] = min(M[k * LD + i], w);
+    for (int q0 = 0; q0 < icnt; q0 += 8 * NT) {
+      int ii[8], kk[8];
+      int32_t ww[8];
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        const int q = ib + min(q0 + r * NT + tid, icnt - 1);
+        ii[r] = c.inner_i[q];
+        kk[r] = c.inner_k[q];
+        ww[r] = c.inner_w[q];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; r++) {
+        if (q0 + r * NT + tid < icnt) {
+          M[ii[r] * LD + kk[r]] = min(M[ii[r] * LD + kk[r]], ww[r]);
+          if (SYM) M[kk[r] * LD + ii[r]] = min(M[kk[r] * LD + ii[r]], ww[r]);
+        }
+      }
     }
   }
   __syncthreads();
