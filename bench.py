@@ -116,6 +116,33 @@ def served_latency(p, n, rate_qps, batches, size, seed):
     return np.array(lat)
 
 
+def query_bytes(tree, lab, x, y):
+    """Algorithmic bytes per query in the reference layout: LCA query
+    84 + 12 w (w = |X(LCA)| + 1), noLCA query 68 + 8 depth(LCA).  The LCA of
+    every pair comes from binary lifting over the father array."""
+    fa = tree["father"].astype(np.int64)
+    dep = tree["depth"].astype(np.int64)
+    root = int(np.nonzero(fa < 0)[0][0])
+    up = [np.where(fa < 0, root, fa)]
+    while (1 << len(up)) < int(dep.max()):
+        up.append(up[-1][up[-1]])
+    a, b = x.astype(np.int64), y.astype(np.int64)
+    swap = dep[a] < dep[b]
+    a, b = np.where(swap, b, a), np.where(swap, a, b)
+    diff = dep[a] - dep[b]
+    for k in range(len(up)):
+        sel = (diff >> k) & 1 == 1
+        a = np.where(sel, up[k][a], a)
+    for k in range(len(up) - 1, -1, -1):
+        na, nb = up[k][a], up[k][b]
+        sel = na != nb
+        a, b = np.where(sel, na, a), np.where(sel, nb, b)
+    lca = np.where(a == b, a, up[0][a])
+    ph = lab["pos_hash"]
+    w = (ph[lca + 1] - ph[lca]).astype(np.int64)
+    return float(np.mean(84 + 12 * w)), float(np.mean(68 + 8 * dep[lca]))
+
+
 def serving_sweep(p, n, peak_qps, requests=400, seed=11):
     """Poisson request arrivals at 10/30/50/70/90% of the plain kernel's peak
     query rate, request sizes log-uniform in 1..65536, into the serving loop
@@ -210,6 +237,70 @@ def scale_run(cores, steps):
     return res
 
 
+def e_scale_run(cores, sources=64, per_band=9200, seed=31):
+    """The E-scale configuration (3.6M vertices, H=14): batched L4 build, and
+    1e7 query pairs mixed by Dijkstra rank: for each of `sources` seeded
+    sources, targets drawn uniformly from the rank bands [2^r, 2^(r+1)),
+    r = 6..22 (ranks from the GPU's own single-source distances), answered
+    in one batched launch and by the plain kernel; two sources are checked
+    against Dijkstra."""
+    g, ids, cut, H = G.make_workload("e3m6")
+    u, v, w = g.arcs()
+    res = {"workload": f"e3m6: {g.n} vertices, {g.m} edges, coordinate bisection H={H}, ChangeToGlobalHeight=1"}
+    rng = np.random.default_rng(seed)
+    with h2h.H2HIndex(0) as p:
+        p.set_graph(g.n, u, v, w, ids, cut, H)
+        res["contraction_device_s"] = p.contract(1, h2h.CONTRACT_GPU_ORDER, cores) * 1e-6
+        entries = p.info()["label_entries"]
+        p.build_labels(h2h.L4, h2h.BATCHED)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            p.build_labels(h2h.L4, h2h.BATCHED)
+        build_s = (time.perf_counter() - t0) / 3
+        res.update({"label_entries": entries, "label_build_s": build_s, "label_entries_per_s": entries / build_s})
+        allv = np.arange(g.n, dtype=np.int32)
+        src = rng.integers(0, g.n, sources).astype(np.int32)
+        xs, ys = [], []
+        for s_ in src:
+            d, _ = p.query(np.full(g.n, s_, np.int32), allv, h2h.QUERY_LCA, h2h.EXEC_BATCHED)
+            order = np.argsort(d, kind="stable")
+            for r in range(6, 23):
+                lo, hi = 1 << r, min(1 << (r + 1), g.n)
+                if lo >= g.n:
+                    break
+                t = order[rng.integers(lo, hi, per_band)]
+                xs.append(np.full(per_band, s_, np.int32))
+                ys.append(t.astype(np.int32))
+        x = np.concatenate(xs)
+        y = np.concatenate(ys)
+        perm = rng.permutation(len(x))
+        x, y = x[perm], y[perm]
+        q = {"pairs": int(len(x))}
+        ref = None
+        for kind, kname in ((h2h.QUERY_LCA, "lca"), (h2h.QUERY_NOLCA, "nolca")):
+            for ex, ename in ((h2h.EXEC_PLAIN, "plain"), (h2h.EXEC_BATCHED, "batched")):
+                best = None
+                for _ in range(3):
+                    r_, us = p.query(x, y, kind, ex)
+                    best = us if best is None else min(best, us)
+                if ref is None:
+                    ref = r_
+                elif not np.array_equal(r_, ref):
+                    raise SystemExit(f"E-scale query variants disagree ({kname}/{ename})")
+                q[f"{kname}_{ename}_qps"] = len(x) / (best * 1e-6)
+        q["batched_over_plain_lca"] = q["lca_batched_qps"] / q["lca_plain_qps"]
+        res["rank_mixed_queries"] = q
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import dijkstra
+        A = csr_matrix((w.astype(np.float64), (u, v)), shape=(g.n, g.n))
+        D = dijkstra(A, indices=src[:2]).astype(np.int64)
+        for i in range(2):
+            d, _ = p.query(np.full(g.n, src[i], np.int32), allv, h2h.QUERY_LCA)
+            if not np.array_equal(d, D[i]):
+                raise SystemExit("E-scale distances differ from Dijkstra")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +309,8 @@ def main():
     ap.add_argument("--workload", default="ny264k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-scale", action="store_true", help="skip the FLA-scale (1.07M vertices) section")
+    ap.add_argument("--e-scale", action="store_true",
+                    help="add the E-scale section (3.6M vertices, 1e7 rank-mixed queries; minutes more)")
     ap.add_argument("--only-build", action="store_true",
                     help="label build steps and the profiled pass only (counter runs)")
     args = ap.parse_args()
@@ -292,6 +385,12 @@ def main():
             if not np.array_equal(r, ref):
                 raise SystemExit(f"query variants disagree ({kname}/{ename})")
             q[f"{kname}_{ename}_qps"] = nq / (min(us) * 1e-6)
+    bq_lca, bq_nolca = query_bytes(p.tree(), lab, x, y)
+    q_roof = {"bytes_per_query_lca": bq_lca, "bytes_per_query_nolca": bq_nolca,
+              "lca_batched_gbs": q["lca_batched_qps"] * bq_lca / 1e9,
+              "nolca_batched_gbs": q["nolca_batched_qps"] * bq_nolca / 1e9,
+              "lca_batched_frac": q["lca_batched_qps"] * bq_lca / 1e9 / HBM_PEAK_GBS,
+              "nolca_batched_frac": q["nolca_batched_qps"] * bq_nolca / 1e9 / HBM_PEAK_GBS}
     # spot check against Dijkstra
     from scipy.sparse import csr_matrix
     from scipy.sparse.csgraph import dijkstra
@@ -359,6 +458,7 @@ def main():
         "contraction_device_s": contract_dev_us * 1e-6,
         "contraction_wall_s": contract_wall,
         "queries": q,
+        "query_roofline": q_roof,
         "batched_over_plain_lca": q["lca_batched_qps"] / q["lca_plain_qps"],
         "latency_4k_ms": {"direct_p50": float(np.percentile(lat_direct, 50)),
                           "direct_p99": float(np.percentile(lat_direct, 99)),
@@ -377,6 +477,8 @@ def main():
     if not args.no_scale and args.workload == "ny264k":
         del p
         out["fla1m"] = scale_run(cpu_cores(), 5)
+    if args.e_scale and args.workload == "ny264k":
+        out["e3m6"] = e_scale_run(cpu_cores())
     print(json.dumps(out))
 
 
