@@ -179,24 +179,39 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
     }
     __syncthreads();
     // ---- fill-in: relax every neighbour pair through v ----
-    for (int p = 0; p < k - 1; p++) {
-      const int u = s_nb[p];
-      const int wu = s_nbw[p];
-      int32_t* rowu = Wp + (int64_t)u * np;
-      for (int q = p + 1 + tid; q < k; q += T) {
-        const int x = s_nb[q];
-        const int ws = wu + s_nbw[q];
-        int32_t* cux = rowu + x;
-        int32_t* cxu = Wp + (int64_t)x * np + u;
-        const int32_t cur = *cux;
-        if (cur == INF32) {
-          *cux = ws;
-          *cxu = ws;
+    // The k x k pair square is spread over the whole workgroup, four pairs
+    // per thread per batch with their loads in flight together; each
+    // unordered pair (p < q) is visited once, so the new-edge test and the
+    // degree increments see no other writer.
+    const int64_t kk = (int64_t)k * k;
+    for (int64_t i0 = tid; i0 < kk; i0 += 4 * T) {
+      int pu[4], px[4], pw[4];
+      int32_t cur[4], cxu[4];
+      bool ok[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t idx = i0 + (int64_t)r * T;
+        const int pp = (int)(idx / max(k, 1)), qq = (int)(idx - (int64_t)pp * k);
+        ok[r] = idx < kk && qq > pp;
+        const int pc = ok[r] ? pp : 0, qc = ok[r] ? qq : 0;
+        pu[r] = s_nb[pc];
+        px[r] = s_nb[qc];
+        pw[r] = s_nbw[pc] + s_nbw[qc];
+        cur[r] = ok[r] ? Wp[(int64_t)pu[r] * np + px[r]] : 0;
+        cxu[r] = ok[r] ? Wp[(int64_t)px[r] * np + pu[r]] : 0;
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (!ok[r]) continue;
+        const int u = pu[r], x = px[r], ws = pw[r];
+        if (cur[r] == INF32) {
+          Wp[(int64_t)u * np + x] = ws;
+          Wp[(int64_t)x * np + u] = ws;
           atomicAdd(&s_deg[u], 1);
           atomicAdd(&s_deg[x], 1);
         } else {
-          if (ws < cur) *cux = ws;
-          if (ws < *cxu) *cxu = ws;
+          if (ws < cur[r]) Wp[(int64_t)u * np + x] = ws;
+          if (ws < cxu[r]) Wp[(int64_t)x * np + u] = ws;
         }
       }
     }
@@ -393,7 +408,8 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     H2H_TRY(up(ctx, D.elim_base, P.elim_base));
     H2H_TRY(up(ctx, D.ncand, P.ncand));
     D.lds = P.max_np <= LDS_STATE_MAX;
-    D.block = (P.height == 0) ? 1024 : 256;
+    // few partitions per level: a whole CU per partition
+    D.block = (P.height == 0 || (int)P.heap.size() < ctx->num_cus) ? 1024 : 256;
     D.shmem = D.lds ? (size_t)6 * P.max_np * sizeof(int32_t) : 0;
     if (!D.lds) H2H_TRY(dev_alloc(ctx, D.gstate, (size_t)6 * P.members.size() * sizeof(int32_t)));
   }
