@@ -285,7 +285,7 @@ struct BagArgs {
   int32_t* bag_w;
 };
 
-__global__ void __launch_bounds__(256) extract_bags_kernel(BagArgs a) {
+__global__ void __launch_bounds__(1024) extract_bags_kernel(BagArgs a) {
   const int part = blockIdx.x;
   const int base = a.memb_off[part];
   const int np = a.memb_off[part + 1] - base;
@@ -477,7 +477,8 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     b.bag_off = d_boff.as<int64_t>();
     b.bag_nbr = d_bnbr.as<int32_t>();
     b.bag_w = d_bw.as<int32_t>();
-    hipLaunchKernelGGL(extract_bags_kernel, dim3((unsigned)ctx->plans[pi].heap.size()), dim3(256), 0, s, b);
+    const int nparts = (int)ctx->plans[pi].heap.size();
+    hipLaunchKernelGGL(extract_bags_kernel, dim3((unsigned)nparts), dim3(nparts < ctx->num_cus ? 1024 : 256), 0, s, b);
     H2H_HIP(ctx, hipGetLastError());
   }
   H2H_HIP(ctx, hipEventRecord(e1, s));

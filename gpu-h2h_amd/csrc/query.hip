@@ -160,15 +160,38 @@ __device__ inline void group_pass(const QArgs& q, const int32_t* __restrict__ x,
       } else {
         const int32_t* dx = q.dis + q.dis_hash[x[qi]];
         const int32_t* dy = q.dis + q.dis_hash[y[qi]];
+        // four strides per round: the position loads, then the eight label
+        // loads, are in flight together (clamped indices stay in the rows)
         if (kind == H2H_QUERY_LCA) {
-          const int64_t e1 = q.pos_hash[L + 1];
-          for (int64_t e = q.pos_hash[L] + g; e < e1; e += G) {
-            const int p = q.pos_pos[e];
-            r = min(r, dx[p] + dy[p]);
+          const int64_t e0 = q.pos_hash[L], e1 = q.pos_hash[L + 1];
+          for (int64_t e = e0 + g; e < e1; e += 4 * G) {
+            int p[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) p[k] = q.pos_pos[min(e + k * G, e1 - 1)];
+            int32_t a[4], b[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              a[k] = dx[p[k]];
+              b[k] = dy[p[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if (e + k * G < e1) r = min(r, a[k] + b[k]);
           }
         } else {
           const int depthL = (int)(q.dis_hash[L + 1] - q.dis_hash[L]);
-          for (int j = g; j < depthL; j += G) r = min(r, dx[j] + dy[j]);
+          for (int j = g; j < depthL; j += 4 * G) {
+            int32_t a[4], b[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+              const int jj = min(j + k * G, depthL - 1);
+              a[k] = dx[jj];
+              b[k] = dy[jj];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+              if (j + k * G < depthL) r = min(r, a[k] + b[k]);
+          }
         }
       }
     }
