@@ -121,6 +121,9 @@ struct Server;
 struct h2h_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // side stream and events of the batched label build (captured into its graph)
+  hipStream_t aux_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join1 = nullptr, ev_join2 = nullptr;
   std::string err;
 
   // ---- graph (Graph::Graph) ----

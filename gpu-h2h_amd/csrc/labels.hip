@@ -29,31 +29,36 @@
 //            rule of kernel_functions.cu:717-744)
 //
 // BATCHED (one hipGraph, replayed)
-//   Maximal subtrees of at most n / (2 * CUs) nodes are persistent tasks of
-//   label_subtree_kernel (one workgroup walks its subtree level by level).
-//   The tree above them is cut into vertical chains of at most 128 nodes (a
-//   chain is a run of nodes each with a single child above the tasks: the
+//   Maximal subtrees of at most TASK_NODES nodes and TASK_H levels are tasks;
+//   the tree above them is cut into vertical chains of at most 128 nodes (a
+//   chain is a run of nodes each with a single child outside the tasks: the
 //   separator vertices of one partition cell).  Chains of equal chain depth
 //   are independent, and a chain c_0..c_{L-1} with head depth h0 splits into
-//     outer columns j < h0-1 (ancestors above the chain): with B_i(j) the
-//       minimum over the bag entries outside the chain,
-//         dis[c_i][j] = min(B_i(j), min over c_k in X(c_i) of w + dis[c_k][j]),
-//       a recurrence down the chain along each column on its own;
+//     outer columns j < h0-1 (ancestors above the chain):
+//         dis[c_i][j] = min over m <= i of P[i][m] + B_m(j)
+//       with B_m(j) the minimum over the bag entries of c_m outside the chain
+//       and P the closure of the entries inside it (upward paths);
 //     inner columns h0-1+m (the chain's own nodes), a triangle:
 //         T[i][m] = min(t_i(m), min over c_k in X(c_i) of
 //                        w + (k >= m ? T[k][m] : L4 ? T[m][k] : inf))
 //       with t_i(m) = min over entries u outside the chain of w + dis[c_m][p_u]
 //       (L4 only).
-//   Four launches per chain depth instead of one per tree level:
-//     chain_bound_kernel  B for every (chain node, 512-column pass), parallel;
-//     chain_fold_kernel   one workgroup per (chain, 128 columns) runs the
-//                         column recurrence in LDS;
-//     chain_cross_kernel  t_i(m), lanes over m (L4);
-//     chain_tri_kernel    one workgroup per chain runs the triangle in LDS.
+//   Per chain depth (a "round"), up to four launches:
+//     bound_kernel        B for every (chain node, 512 columns);
+//     chain_apply_kernel  P (x) B for every (chain, 64 columns);
+//     chain_cross_kernel  t_i(m) for every chain node (L4);
+//     chain_fw_kernel     the triangle by Floyd-Warshall, one workgroup per
+//                         chain.  (The chain nodes' bags hold most of the
+//                         chain above them, so the row recurrence would be
+//                         as much work, run by one wavefront.)
+//   The closures P of all chains are one more launch, ahead of the rounds.
+//   A task's rows split the same way at its root: B from the entries above
+//   the task (bound_kernel), P from task_close_kernel, the outer columns by
+//   task_apply_kernel and the inner columns level by level inside one
+//   workgroup per task (task_inner_kernel).
 //   The L4 gathers dis[a_j][p] of chain and task rows read a transposed copy
-//   of the chain rows (tb[p][t] = dis[chain node t][p], written by the fold
-//   and triangle kernels) so that the 64 lanes of a gather hit consecutive
-//   words.
+//   of the chain rows (tb[p][t] = dis[chain node t][p], written with the
+//   rows) so that the 64 lanes of a gather hit consecutive words.
 //
 // A workgroup that waits for another workgroup's row inside one launch needs
 // a device-scope release per row, which on this part writes back the L2;
@@ -739,8 +744,8 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
     c.begin = rd.k2b;
     c.count = rd.k2n;
     const size_t lds = apply_lds(rd.lmax);
-    if (i32) hipLaunchKernelGGL((chain_apply_kernel<true>), xcd_grid(rd.k2n), dim3(256), lds, s, c);
-    else hipLaunchKernelGGL((chain_apply_kernel<false>), xcd_grid(rd.k2n), dim3(256), lds, s, c);
+    if (i32) hipLaunchKernelGGL((chain_apply_kernel<true>), xcd_grid(c.count), dim3(256), lds, s, c);
+    else hipLaunchKernelGGL((chain_apply_kernel<false>), xcd_grid(c.count), dim3(256), lds, s, c);
   } else if (step == 2 && rd.k3n > 0 && l4) {
     c.begin = rd.k3b;
     c.count = rd.k3n;
@@ -1480,14 +1485,6 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     c.tro = P.tro;
     c.pro = P.pro;
     c.begin = c.count = 0;
-    if (P.kcl_n > 0) H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b, P.kcl_n, P.kcl_lmax, s); }));
-    for (const auto& rd : P.rounds) {
-      const int n[4] = {rd.k1n, rd.k2n, a.l4 ? rd.k3n : 0, rd.k4n};
-      for (int step = 0; step < 4; step++)
-        if (n[step] > 0) H2H_TRY(timed([&] { launch_chain_round(c, rd, s, step); }));
-    }
-  }
-  if (P.ntasks > 0) {
     TaskArgs t;
     t.a = a;
     t.lvl_base = P.task_lvl_base.as<int32_t>();
@@ -1495,6 +1492,7 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.lvl_depth = P.task_lvl_depth.as<int32_t>();
     t.tbase = P.task_tbase.as<int32_t>();
     t.troot = P.task_troot.as<int32_t>();
+    t.tinfo = P.task_tinfo.as<int4>();
     t.bnodes = P.task_nodes.as<int32_t>();
     t.banc = P.task_banc.as<int32_t>();
     t.tanc = P.task_tanc.as<int64_t>();
@@ -1508,21 +1506,52 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.ubegin = 0;
     t.nunits = P.task_nunits;
     t.ntasks = P.ntasks;
-    t.tinfo = P.task_tinfo.as<int4>();
-    if (P.task_levels) {
-      H2H_TRY(timed([&] { launch_task_levels(t, P.tpath.as<int32_t>(), s); }));
-      return H2H_OK;
+    const bool task_close = P.ntasks > 0 && !P.task_levels;
+    // The two closures depend only on the tree's bag weights: off the timed
+    // path (ev == nullptr) they run on a side stream beside the chain rounds
+    // and are joined where their results are first read -- the chains'
+    // before the first outer-column apply, the tasks' before the task passes.
+    hipStream_t s2 = ev ? nullptr : ctx->aux_stream;
+    if (s2) {
+      H2H_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+      H2H_HIP(ctx, hipStreamWaitEvent(s2, ctx->ev_fork, 0));
+      if (P.kcl_n > 0) launch_chain_closure(c, P.kcl_b, P.kcl_n, P.kcl_lmax, s2);
+      H2H_HIP(ctx, hipEventRecord(ctx->ev_join1, s2));
+      if (task_close) launch_tasks(t, s2, 0);
+      H2H_HIP(ctx, hipEventRecord(ctx->ev_join2, s2));
+    } else if (P.kcl_n > 0) {
+      H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b, P.kcl_n, P.kcl_lmax, s); }));
     }
-    H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
-    for (size_t k = 0; k + 1 < P.tchunk_b.size(); k++) {
-      const int b0 = P.tchunk_b[k], b1 = P.tchunk_b[k + 1];
-      if (b1 > b0)
-        H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), b0, b1 - b0, s, false); }));
-      t.ubegin = P.tchunk_a[k];
-      t.nunits = P.tchunk_a[k + 1] - P.tchunk_a[k];
-      if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+    bool joined1 = s2 == nullptr;
+    for (const auto& rd : P.rounds) {
+      const int n[4] = {rd.k1n, rd.k2n, a.l4 ? rd.k3n : 0, rd.k4n};
+      for (int step = 0; step < 4; step++) {
+        if (n[step] == 0) continue;
+        if (step == 1 && !joined1) {
+          H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join1, 0));
+          joined1 = true;
+        }
+        H2H_TRY(timed([&] { launch_chain_round(c, rd, s, step); }));
+      }
     }
-    H2H_TRY(timed([&] { launch_tasks(t, s, 2); }));
+    if (!joined1) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join1, 0));
+    if (s2) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join2, 0));
+    if (P.ntasks > 0) {
+      if (P.task_levels) {
+        H2H_TRY(timed([&] { launch_task_levels(t, P.tpath.as<int32_t>(), s); }));
+        return H2H_OK;
+      }
+      if (!s2) H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
+      for (size_t k = 0; k + 1 < P.tchunk_b.size(); k++) {
+        const int b0 = P.tchunk_b[k], b1 = P.tchunk_b[k + 1];
+        if (b1 > b0)
+          H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), b0, b1 - b0, s, false); }));
+        t.ubegin = P.tchunk_a[k];
+        t.nunits = P.tchunk_a[k + 1] - P.tchunk_a[k];
+        if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+      }
+      H2H_TRY(timed([&] { launch_tasks(t, s, 2); }));
+    }
   }
   return H2H_OK;
 }
@@ -1537,7 +1566,12 @@ h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, Label
   const bool anc = l4 && method != H2H_METHOD_BATCHED;  // ancestor rows: the level kernels only
   if (anc) H2H_TRY(dev_alloc(ctx, ctx->d_anc, std::max<int64_t>(ctx->label_entries, 1) * sizeof(uint32_t)));
   H2H_TRY(dev_alloc(ctx, ctx->d_dis, std::max<int64_t>(P.dis_entries, 1) * sizeof(int32_t)));
-  if (method == H2H_METHOD_BATCHED) H2H_HIP(ctx, set_chain_lds_limits());
+  if (method == H2H_METHOD_BATCHED) {
+    H2H_HIP(ctx, set_chain_lds_limits());
+    if (!ctx->aux_stream) H2H_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
+    for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_join1, &ctx->ev_join2})
+      if (!*e) H2H_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
+  }
   a.bfs = ctx->d_bfs.as<int32_t>();
   a.father = ctx->d_father.as<int32_t>();
   a.dis_hash = ctx->d_dis_hash.as<int64_t>();

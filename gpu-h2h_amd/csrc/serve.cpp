@@ -177,6 +177,9 @@ h2h_ctx::h2h_ctx() = default;
 
 h2h_ctx::~h2h_ctx() {
   server.reset();
+  for (hipEvent_t e : {ev_fork, ev_join1, ev_join2})
+    if (e) (void)hipEventDestroy(e);
+  if (aux_stream) (void)hipStreamDestroy(aux_stream);
   if (stream) (void)hipStreamDestroy(stream);
 }
 
