@@ -320,6 +320,7 @@ h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, c
   }
   ctx->bag_nbr.assign(ctx->bag_off[n], 0);
   ctx->bag_w.assign(ctx->bag_off[n], 0);
+  ctx->bag_dom.clear();  // no contraction: no two-hop marks
   int head = -1;
   for (int v = 0; v < n; v++) {
     if (father[v] < 0) {

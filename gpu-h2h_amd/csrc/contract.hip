@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <numeric>
+#include <tuple>
 
 #include "h2h_internal.h"
 
@@ -33,6 +34,8 @@ namespace {
 constexpr int F_ALIVE = 1;
 constexpr int F_CAND = 2;
 constexpr int LDS_STATE_MAX = 2048;  // members whose state fits in LDS
+constexpr int STATE_WORDS = 7;       // per-member state words
+constexpr int DOM_MIN_BAG = 32;      // two-hop marks only for bags this large
 
 struct ContractArgs {
   const int32_t* memb_off;   // [parts+1]
@@ -71,6 +74,57 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
   return v;
 }
 
+// Fill-in through the eliminated vertex: every neighbour pair (u, x) gets
+// w(u, x) = min(w(u, x), w(v, u) + w(v, x)).  The k x k pair square is
+// spread over the whole workgroup, four pairs per thread per batch with
+// their loads in flight together; each unordered pair (p < q) is visited
+// once, so the new-edge test and the degree increments see no other writer.
+// MARK: the same pass flags the members whose entry in v's bag a two-hop
+// path through another bag member beats strictly (w(v,x) + w(x,u) <
+// w(v,u)).  Any value w(x,u) takes is the length of a path, so such an entry
+// never gives the minimum of an exact-distance label recurrence.
+template <bool MARK>
+__device__ inline void fill_in(int k, int np, int tid, int T, int32_t* Wp, const int32_t* s_nb, const int32_t* s_nbw,
+                               int32_t* s_deg, int32_t* s_dom) {
+  const int64_t kk = (int64_t)k * k;
+  for (int64_t i0 = tid; i0 < kk; i0 += 4 * T) {
+    int pu[4], px[4], wu[4], wx[4];
+    int32_t cur[4], cxu[4];
+    bool ok[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int64_t idx = i0 + (int64_t)r * T;
+      const int pp = (int)(idx / max(k, 1)), qq = (int)(idx - (int64_t)pp * k);
+      ok[r] = idx < kk && qq > pp;
+      const int pc = ok[r] ? pp : 0, qc = ok[r] ? qq : 0;
+      pu[r] = s_nb[pc];
+      px[r] = s_nb[qc];
+      wu[r] = s_nbw[pc];
+      wx[r] = s_nbw[qc];
+      cur[r] = ok[r] ? Wp[(int64_t)pu[r] * np + px[r]] : 0;
+      cxu[r] = ok[r] ? Wp[(int64_t)px[r] * np + pu[r]] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (!ok[r]) continue;
+      const int u = pu[r], x = px[r], ws = wu[r] + wx[r];
+      if (MARK) {  // non-negative int32 weights: the uint32 sums cannot wrap
+        if ((uint32_t)wx[r] + (uint32_t)cxu[r] < (uint32_t)wu[r]) s_dom[u] = 1;
+        if ((uint32_t)wu[r] + (uint32_t)cur[r] < (uint32_t)wx[r]) s_dom[x] = 1;
+      }
+      if (cur[r] == INF32) {
+        Wp[(int64_t)u * np + x] = ws;
+        Wp[(int64_t)x * np + u] = ws;
+        atomicAdd(&s_deg[u], 1);
+        atomicAdd(&s_deg[x], 1);
+      } else {
+        if (ws < cur[r]) Wp[(int64_t)u * np + x] = ws;
+        if (ws < cxu[r]) Wp[(int64_t)x * np + u] = ws;
+      }
+    }
+  }
+}
+
 __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
   extern __shared__ int32_t smem[];
   __shared__ uint64_t red[16];
@@ -86,13 +140,14 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
   const int nwaves = T >> 6;
   int32_t* Wp = a.W + a.w_off[part];
 
-  int32_t* st = a.state_in_lds ? smem : (a.gstate + 6LL * base);
+  int32_t* st = a.state_in_lds ? smem : (a.gstate + (int64_t)STATE_WORDS * base);
   int32_t* s_deg = st;
   int32_t* s_hgt = st + np;
   int32_t* s_flag = st + 2 * np;
   int32_t* s_holes = st + 3 * np;
   int32_t* s_nb = st + 4 * np;
   int32_t* s_nbw = st + 5 * np;
+  int32_t* s_dom = st + 6 * np;  // per member: its entry in v's bag is beaten by a two-hop path
 
   // ---- load: dense block from the current adjacency rows ----
   const int64_t nn = (int64_t)np * np;
@@ -172,6 +227,7 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
     // ---- remove v from its neighbours; height propagation ----
     for (int t = tid; t < k; t += T) {
       const int u = s_nb[t];
+      s_dom[u] = 0;
       const int d = s_deg[u] - 1;
       s_deg[u] = d;
       if (a.track_holes) s_holes[u] = (d == 0) ? s_holes[u] + 1 : 0;
@@ -179,43 +235,17 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
     }
     __syncthreads();
     // ---- fill-in: relax every neighbour pair through v ----
-    // The k x k pair square is spread over the whole workgroup, four pairs
-    // per thread per batch with their loads in flight together; each
-    // unordered pair (p < q) is visited once, so the new-edge test and the
-    // degree increments see no other writer.
-    const int64_t kk = (int64_t)k * k;
-    for (int64_t i0 = tid; i0 < kk; i0 += 4 * T) {
-      int pu[4], px[4], pw[4];
-      int32_t cur[4], cxu[4];
-      bool ok[4];
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const int64_t idx = i0 + (int64_t)r * T;
-        const int pp = (int)(idx / max(k, 1)), qq = (int)(idx - (int64_t)pp * k);
-        ok[r] = idx < kk && qq > pp;
-        const int pc = ok[r] ? pp : 0, qc = ok[r] ? qq : 0;
-        pu[r] = s_nb[pc];
-        px[r] = s_nb[qc];
-        pw[r] = s_nbw[pc] + s_nbw[qc];
-        cur[r] = ok[r] ? Wp[(int64_t)pu[r] * np + px[r]] : 0;
-        cxu[r] = ok[r] ? Wp[(int64_t)px[r] * np + pu[r]] : 0;
-      }
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        if (!ok[r]) continue;
-        const int u = pu[r], x = px[r], ws = pw[r];
-        if (cur[r] == INF32) {
-          Wp[(int64_t)u * np + x] = ws;
-          Wp[(int64_t)x * np + u] = ws;
-          atomicAdd(&s_deg[u], 1);
-          atomicAdd(&s_deg[x], 1);
-        } else {
-          if (ws < cur[r]) Wp[(int64_t)u * np + x] = ws;
-          if (ws < cxu[r]) Wp[(int64_t)x * np + u] = ws;
-        }
-      }
+    if (k >= DOM_MIN_BAG) {
+      fill_in<true>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
+      __syncthreads();
+      // v's row is frozen from here on: a dominated entry is stored there
+      // as -(w + 1) for the bag extraction
+      for (int t = tid; t < k; t += T)
+        if (s_dom[s_nb[t]]) Wp[(int64_t)v * np + s_nb[t]] = -s_nbw[t] - 1;
+    } else {
+      fill_in<false>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   // ---- write back the surviving (boundary) members' adjacency rows ----
@@ -410,8 +440,8 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     D.lds = P.max_np <= LDS_STATE_MAX;
     // few partitions per level: a whole CU per partition
     D.block = (P.height == 0 || (int)P.heap.size() < ctx->num_cus) ? 1024 : 256;
-    D.shmem = D.lds ? (size_t)6 * P.max_np * sizeof(int32_t) : 0;
-    if (!D.lds) H2H_TRY(dev_alloc(ctx, D.gstate, (size_t)6 * P.members.size() * sizeof(int32_t)));
+    D.shmem = D.lds ? (size_t)STATE_WORDS * P.max_np * sizeof(int32_t) : 0;
+    if (!D.lds) H2H_TRY(dev_alloc(ctx, D.gstate, (size_t)STATE_WORDS * P.members.size() * sizeof(int32_t)));
   }
 
   hipEvent_t e0, e1;
@@ -499,6 +529,13 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
   (void)hipEventDestroy(e1);
   ctx->t_contract_us = ms * 1000.0;
   if (usec) *usec = ctx->t_contract_us;
+  // dominated entries come back as -(w + 1)
+  ctx->bag_dom.assign(nb, 0);
+  for (int64_t e = 0; e < nb; e++)
+    if (ctx->bag_w[e] < 0) {
+      ctx->bag_dom[e] = 1;
+      ctx->bag_w[e] = -ctx->bag_w[e] - 1;
+    }
   // sort each bag by neighbour id (the device writes them in column order,
   // which is ascending id inside a partition already)
   for (int v = 0; v < n; v++) {
@@ -507,10 +544,10 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     for (int64_t e = b0 + 1; e < b1; e++)
       if (ctx->bag_nbr[e - 1] > ctx->bag_nbr[e]) { sorted = false; break; }
     if (!sorted) {
-      std::vector<std::pair<int32_t, int32_t>> t;
-      for (int64_t e = b0; e < b1; e++) t.emplace_back(ctx->bag_nbr[e], ctx->bag_w[e]);
+      std::vector<std::tuple<int32_t, int32_t, uint8_t>> t;
+      for (int64_t e = b0; e < b1; e++) t.emplace_back(ctx->bag_nbr[e], ctx->bag_w[e], ctx->bag_dom[e]);
       std::sort(t.begin(), t.end());
-      for (int64_t e = b0; e < b1; e++) { ctx->bag_nbr[e] = t[e - b0].first; ctx->bag_w[e] = t[e - b0].second; }
+      for (int64_t e = b0; e < b1; e++) std::tie(ctx->bag_nbr[e], ctx->bag_w[e], ctx->bag_dom[e]) = t[e - b0];
     }
   }
   return H2H_OK;

@@ -76,6 +76,8 @@ struct LabelPlan {
   DevArray task_tbase, task_troot, task_banc, task_tanc, task_tanl, task_bn_ie, task_ie_s, task_ie_w;
   DevArray task_cl_w, task_cl_ro, task_units, bunits;  // closure lists; bound units of chains and tasks
   DevArray task_tinfo;
+  DevArray bpos_pos, bpos_w, bpos_rowoff;  // batched plan's entry copy: kept outer entries, dropped ones, the rest
+  bool pruned = false;
   bool task_levels = false;  // tasks by levels (one launch) or by closure (bound, close, apply, inner)
   int task_nunits = 0, tbound_b = 0, tbound_n = 0;
   std::vector<int> tchunk_b, tchunk_a;  // task chunks: bound and apply unit offsets
@@ -105,6 +107,7 @@ struct LabelPlan {
     ntasks = 0;
     task_nunits = tbound_b = tbound_n = 0;
     task_nodes_n = 0;
+    pruned = false;
     rounds.clear();
     nt = tbo = tro = pro = dis_entries = 0;
     kcl_b = kcl_n = 0;
@@ -153,10 +156,12 @@ struct h2h_ctx {
   // ---- tree and label layout (host copies) ----
   std::vector<int64_t> bag_off;   // [n+1]
   std::vector<int32_t> bag_nbr, bag_w;
+  std::vector<uint8_t> bag_dom;   // 1: entry beaten by a two-hop path through the bag (empty: unknown)
   std::vector<int32_t> rank, father, depth, first_appear, bfs, level_off;
   std::vector<int32_t> preorder, tin;
   std::vector<int64_t> dis_hash, pos_hash;
   std::vector<int32_t> pos_id, pos_pos, pos_w;
+  std::vector<uint8_t> pos_dom;   // bag_dom in pos order (empty: unknown)
   int32_t head = -1;
   int32_t tree_height = 0;
   int64_t label_entries = 0;

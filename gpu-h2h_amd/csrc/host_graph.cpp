@@ -292,12 +292,15 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
   ctx->pos_id.assign(P, 0);
   ctx->pos_pos.assign(P, 0);
   ctx->pos_w.assign(P, 0);
-  std::vector<std::tuple<int32_t, int32_t, int32_t>> ent;
+  const bool dom = ctx->bag_dom.size() == ctx->bag_nbr.size() && !ctx->bag_dom.empty();
+  if (dom) ctx->pos_dom.assign(P, 0);
+  else ctx->pos_dom.clear();
+  std::vector<std::tuple<int32_t, int32_t, int32_t, uint8_t>> ent;
   for (int v = 0; v < n; v++) {
     ent.clear();
     for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
       int32_t u = ctx->bag_nbr[e];
-      ent.emplace_back(ctx->depth[u] - 1, u, ctx->bag_w[e]);
+      ent.emplace_back(ctx->depth[u] - 1, u, ctx->bag_w[e], dom ? ctx->bag_dom[e] : 0);
     }
     std::sort(ent.begin(), ent.end());
     int64_t k = ctx->pos_hash[v];
@@ -305,6 +308,7 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
       ctx->pos_pos[k] = std::get<0>(t);
       ctx->pos_id[k] = std::get<1>(t);
       ctx->pos_w[k] = std::get<2>(t);
+      if (dom) ctx->pos_dom[k] = std::get<3>(t);
       k++;
     }
     ctx->pos_pos[k] = ctx->depth[v] - 1;

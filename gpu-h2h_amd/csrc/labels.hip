@@ -329,12 +329,13 @@ inline dim3 xcd_grid(int nblocks) { return dim3((unsigned)(((nblocks + XCDS - 1)
 
 // Bounds from the bag entries above a node's chain or task: one unit =
 // columns [c0, c0 + 512) of the outer columns [0, L) of vertex v, from its
-// first `nouter` bag entries (positions below L).  The ancestor a_j of column
+// first `nouter` bag entries (positions below L; L4 reads the first
+// `nouter4`, the entries no two-hop path through the bag beats).  The ancestor a_j of column
 // j has chain index j < ch0 - 1 ? tpath[path + j] : cfirst + j - (ch0 - 1)
 // (the chain holding v's first ancestor above the chain or task).
 struct BoundUnit {
   int32_t v, c0, L, nouter;
-  int32_t path, cfirst, ch0, pad;
+  int32_t path, cfirst, ch0, nouter4;  // nouter4: the entries L4 reads (two-hop-dominated ones left out)
 };
 
 template <bool L4, bool IDX32, bool SPLIT>
@@ -358,8 +359,9 @@ __global__ void __launch_bounds__(256) bound_kernel(LabelArgs a, const BoundUnit
     g[r] = L4 ? (jj < b.ch0 - 1 ? (uint32_t)tpath[b.path + jj] : (uint32_t)(b.cfirst + jj - (b.ch0 - 1))) : 0u;
   }
   const int64_t e0 = a.pos_hash[b.v];
+  const int nouter = L4 ? b.nouter4 : b.nouter;
   if (SPLIT) {
-    const int nk = b.nouter > wid ? (b.nouter - wid + 3) / 4 : 0;
+    const int nk = nouter > wid ? (nouter - wid + 3) / 4 : 0;
     accumulate<L4, IDX32, true>(a, e0 + wid, 4, nk, b.c0, b.L, lane, g, acc);
 #pragma unroll
     for (int r = 0; r < R; r++) comb[(wid * R + r) * 64 + lane] = acc[r];
@@ -375,7 +377,7 @@ __global__ void __launch_bounds__(256) bound_kernel(LabelArgs a, const BoundUnit
     }
     return;
   }
-  accumulate<L4, IDX32, true>(a, e0, 1, b.nouter, b.c0, b.L, lane, g, acc);
+  accumulate<L4, IDX32, true>(a, e0, 1, nouter, b.c0, b.L, lane, g, acc);
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int j = b.c0 + r * 64 + lane;
@@ -1206,7 +1208,13 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   }
   const int64_t nt = (int64_t)tnode.size();
   std::vector<Chain> ch(corder.size());
-  std::vector<int32_t> tpath, touter(nt), tinner(nt + 1, 0), inner_w;
+  // L4 leaves out the outer entries that a two-hop path through the bag
+  // beats (marked by the contraction); the entry copy the batched kernels
+  // read puts, per node, the kept outer entries first, then the dropped
+  // ones, then the rest, so both kinds read a prefix
+  const bool prune = !ctx->pos_dom.empty() && env_int("H2H_PRUNE", 1, 0, 1) != 0;
+  std::vector<int32_t> tpath, touter(nt), touter4(nt, 0), tinner(nt + 1, 0), inner_w;
+  std::vector<int32_t> thr(n, 0);  // outer-entry threshold of every node (positions below it)
   std::vector<uint8_t> inner_k, inner_i;
   int64_t tri = 0;
   int maxdepth = 0;
@@ -1237,6 +1245,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       int64_t e = e0;
       while (e < e1 && ctx->pos_pos[e] < h) e++;
       touter[t] = (int32_t)(e - e0);
+      thr[v] = h;
+      for (int64_t q = e0; q < e; q++) touter4[t] += prune && ctx->pos_dom[q] ? 0 : 1;
       for (; e < e1; e++) {
         inner_w.push_back(ctx->pos_w[e]);
         inner_k.push_back((uint8_t)(ctx->pos_pos[e] - h));
@@ -1268,7 +1278,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           b.path = ch[ci].path;
           b.cfirst = ch[ci].first;
           b.ch0 = ch[ci].h0;
-          b.pad = 0;
+          b.nouter4 = touter4[ch[ci].first + i];
           bunits.push_back(b);
         }
     }
@@ -1298,7 +1308,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   for (const auto& x : ch) P.kcl_lmax = std::max(P.kcl_lmax, (int)x.len);
   H2H_TRY(upload(ctx, P.chains, ch.data(), ch.size() * sizeof(Chain)));
   H2H_TRY(upload(ctx, P.tnode, tnode.data(), tnode.size() * sizeof(int32_t)));
-  H2H_TRY(upload(ctx, P.touter, touter.data(), touter.size() * sizeof(int32_t)));
+  H2H_TRY(upload(ctx, P.touter, touter4.data(), touter4.size() * sizeof(int32_t)));  // read by the L4 cross only
   H2H_TRY(upload(ctx, P.tinner, tinner.data(), tinner.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.inner_w, inner_w.data(), inner_w.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.inner_k, inner_k.data(), inner_k.size()));
@@ -1343,8 +1353,12 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         const int v = mem[k];
         const int d = ctx->depth[v] - r;
         const int kg = (int)bnodes.size();  // BFS node index
-        int nouter = 0;
-        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1 && ctx->pos_pos[e] < r - 1; e++) nouter++;
+        thr[v] = r - 1;
+        int nouter = 0, nouter4 = 0;
+        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1 && ctx->pos_pos[e] < r - 1; e++) {
+          nouter++;
+          nouter4 += prune && ctx->pos_dom[e] ? 0 : 1;
+        }
         for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
           BoundUnit b;
           b.v = v;
@@ -1354,7 +1368,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           b.path = fc.path;
           b.cfirst = fc.first;
           b.ch0 = fc.h0;
-          b.pad = 0;
+          b.nouter4 = nouter4;
           tb_units.push_back(b);
           ta_units.push_back(make_int4(kg, q0, r - 1, d + 1));
         }
@@ -1422,6 +1436,29 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     P.task_nunits = (int)aunits.size();
   }
   H2H_TRY(upload(ctx, P.bunits, bunits.data(), bunits.size() * sizeof(BoundUnit)));
+  P.pruned = prune;
+  if (prune) {
+    const int64_t PE = ctx->pos_hash[n];
+    std::vector<int32_t> bp(PE), bw(PE);
+    std::vector<int64_t> br(PE);
+    for (int v = 0; v < n; v++) {
+      const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1];
+      int64_t k = e0;
+      for (int pass = 0; pass < 3; pass++)
+        for (int64_t e = e0; e < e1; e++) {
+          const bool outer = e < e1 - 1 && ctx->pos_pos[e] < thr[v];
+          const int cls = outer ? (ctx->pos_dom[e] ? 1 : 0) : 2;
+          if (cls != pass) continue;
+          bp[k] = ctx->pos_pos[e];
+          bw[k] = ctx->pos_w[e];
+          br[k] = ctx->dis_hash[ctx->pos_id[e]];
+          k++;
+        }
+    }
+    H2H_TRY(upload(ctx, P.bpos_pos, bp.data(), PE * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.bpos_w, bw.data(), PE * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.bpos_rowoff, br.data(), PE * sizeof(int64_t)));
+  }
   auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };
   P.nt = nt;
   P.tbo = up64(ctx->label_entries);
@@ -1579,6 +1616,11 @@ h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, Label
   a.pos_pos = ctx->d_pos_pos.as<int32_t>();
   a.pos_w = ctx->d_pos_w.as<int32_t>();
   a.pos_rowoff = ctx->d_pos_rowoff.as<int64_t>();
+  if (method == H2H_METHOD_BATCHED && P.pruned) {
+    a.pos_pos = P.bpos_pos.as<int32_t>();
+    a.pos_w = P.bpos_w.as<int32_t>();
+    a.pos_rowoff = P.bpos_rowoff.as<int64_t>();
+  }
   a.dis = ctx->d_dis.as<int32_t>();
   a.anc = anc ? ctx->d_anc.as<uint32_t>() : nullptr;
   a.l4 = l4 ? 1 : 0;
