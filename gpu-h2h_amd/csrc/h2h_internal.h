@@ -89,7 +89,8 @@ struct LabelPlan {
     int k1b, k1n, k2b, k2n, k3b, k3n, k4b, k4n, lmax, imax;
   };
   std::vector<Round> rounds;
-  int kcl_b = 0, kcl_n = 0, kcl_lmax = 0;  // units of the closure launch (every chain)
+  int kcl_b[3] = {0, 0, 0}, kcl_n[3] = {0, 0, 0};  // closure launches (every chain) by size class 128, 64, 32
+  int tclose_after = 0;                            // chain depths before the task closures start
   int64_t nt = 0;           // nodes in chains
   int64_t tbo = 0;          // offset of the transposed chain rows inside dis
   int64_t tro = 0;          // offset of the t_i(m) triangles (scratch) inside dis
@@ -110,7 +111,8 @@ struct LabelPlan {
     pruned = false;
     rounds.clear();
     nt = tbo = tro = pro = dis_entries = 0;
-    kcl_b = kcl_n = 0;
+    for (int k = 0; k < 3; k++) kcl_b[k] = kcl_n[k] = 0;
+    tclose_after = 0;
     graph_dis[0] = graph_dis[1] = nullptr;
   }
   ~LabelPlan() { reset(); }

@@ -317,6 +317,7 @@ struct ChainArgs {
   int64_t tro;             // t_i(m) triangles (scratch inside dis)
   int64_t pro;             // closure triangles P (scratch inside dis)
   int begin, count;        // units of this launch
+  int tri;                 // L4 triangles as the min-plus product (else Floyd-Warshall)
 };
 
 // blockIdx -> work index so that each XCD takes a contiguous run (see above)
@@ -497,13 +498,16 @@ __global__ void __launch_bounds__(256) chain_apply_kernel(ChainArgs c) {
   }
 }
 
-// t_i(m) = min over the entries of c_i outside the chain of w + dis[c_m][p]
-// (L4), lanes over m, read from the transposed rows; written with the
-// diagonal 0 into row i of the chain's triangle scratch.
-template <bool IDX32>
+// Cross terms between a chain's nodes through the entries above the chain
+// (L4), read from the transposed rows.  One workgroup per chain node x; its
+// four wavefronts split x's outer entries (stride 4) and combine through LDS.
+//   UP = false: t_x(y) = min over the outer entries u of x of
+//               w + dis[c_y][p_u] for y <= x (lanes over y), written with the
+//               diagonal 0 into row x of the chain's triangle scratch;
+//   UP = true:  the same sums for y > x, written to entry (y, x) of the
+//               scratch (row y, column x < y).
+template <bool IDX32, bool UP>
 __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
-  // one workgroup per chain node; its four wavefronts split the entries
-  // (stride 4) and combine through LDS
   __shared__ int32_t comb[4][128];
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int u = xcd_index(c.count);
@@ -516,8 +520,10 @@ __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
   const int v = c.tnode[t];
   const int nk = c.touter[t];
   const int64_t e0 = a.pos_hash[v];
-  const int m0 = min(lane, n - 1), m1 = min(lane + 64, n - 1);
-  const bool two = i > 64;
+  // lanes' chain positions y (clamped: always a valid address)
+  const int m0 = UP ? min(i + 1 + lane, n - 1) : min(lane, n - 1);
+  const int m1 = UP ? min(i + 65 + lane, n - 1) : min(lane + 64, n - 1);
+  const bool two = UP ? n - 1 - i > 64 : i > 64;
   int32_t x0 = INF32, x1 = INF32;
   for (int kb = wid * 64; kb < nk; kb += 256) {
     int mp = 0, mw = 0;
@@ -557,9 +563,15 @@ __global__ void __launch_bounds__(256) chain_cross_kernel(ChainArgs c) {
   if (wid == 0) {
     x0 = min(min(comb[0][lane], comb[1][lane]), min(comb[2][lane], comb[3][lane]));
     x1 = min(min(comb[0][lane + 64], comb[1][lane + 64]), min(comb[2][lane + 64], comb[3][lane + 64]));
-    const uint64_t trow = (uint64_t)c.tro + ch.tri + (uint64_t)i * (i + 1) / 2;
-    if (lane <= i) a.dis[trow + lane] = lane == i ? 0 : x0;
-    if (lane + 64 <= i) a.dis[trow + lane + 64] = lane + 64 == i ? 0 : x1;
+    if (UP) {
+      const int y0 = i + 1 + lane, y1 = i + 65 + lane;
+      if (y0 < n) a.dis[(uint64_t)c.tro + ch.tri + (uint64_t)y0 * (y0 + 1) / 2 + i] = x0;
+      if (y1 < n) a.dis[(uint64_t)c.tro + ch.tri + (uint64_t)y1 * (y1 + 1) / 2 + i] = x1;
+    } else {
+      const uint64_t trow = (uint64_t)c.tro + ch.tri + (uint64_t)i * (i + 1) / 2;
+      if (lane <= i) a.dis[trow + lane] = lane == i ? 0 : x0;
+      if (lane + 64 <= i) a.dis[trow + lane + 64] = lane + 64 == i ? 0 : x1;
+    }
   }
 }
 
@@ -704,6 +716,132 @@ __global__ void __launch_bounds__(fw_threads(S)) chain_fw_kernel(ChainArgs c) {
   }
 }
 
+// L4 triangle as one min-plus product.  With P the closure of the chain's
+// inner entries (upward paths, P[m][m] = 0) and t'_q(i) the cross sums
+// through q's entries above the chain (chain_cross_kernel<UP>),
+//   T[i][m] = min over q <= m of P[m][q] + B[i][q],
+//   B[i][q] = min(P[i][q], t'_q(i))                       (q < i).
+// Take a shortest path between c_i and c_m (m < i) and its highest vertex z.
+// If z = c_m it is an upward path from c_i, P[i][m].  Otherwise it leaves
+// c_m through a bag member u of c_m, so d(c_m, c_i) = min over u in X(c_m)
+// of w(c_m, u) + d(u, c_i): an entry above the chain gives t'_m(i), one
+// inside it (u = c_q, q < m) gives w + T[i][q].  Unrolling that recurrence
+// over the inner entries gives the product with the closure P.  No pivot
+// order: the triangle is one pass of 4 x 4 register tiles.
+//   LDS: P and B by q (Pt[q][m] = P[m][q], Bt[q][i] = B[i][q]), row q kept
+//   from column 4 floor(q / 4) on (tri_row_off), and the result M.
+//   Work items: (tile column mb, 32-step slice c of q < 4 mb + 4, tile row
+//   ib >= mb), one per thread; neighbouring lanes share (mb, c), so a
+//   wavefront's lanes run loops of one length.  Slices of one tile combine
+//   with LDS atomicMin.
+constexpr int TRI_CQ = 32;  // q steps per work item
+__host__ __device__ constexpr int tri_rows_words(int S) { return S * S - 2 * S * (S / 4 - 1); }
+__device__ inline int tri_row_off(int S, int q) {
+  const int a = q >> 2, b = q & 3;
+  return 4 * a * S - 8 * a * (a - 1) + b * (S - 4 * a);
+}
+size_t tri_lds(int S) { return (size_t)(2 * tri_rows_words(S) + S * (S + 1)) * 4; }
+
+template <int S, bool IDX32>
+__global__ void __launch_bounds__(fw_threads(S)) chain_tri_kernel(ChainArgs c) {
+  constexpr int NT = fw_threads(S), LD = S + 1, NB = S / 4, RW = tri_rows_words(S);
+  extern __shared__ int32_t lds[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int u = xcd_index(c.count);
+  if (u >= c.count) return;  // whole workgroup
+  const LabelArgs& a = c.a;
+  const Chain ch = c.chains[c.units[c.begin + u].x];
+  const int n = ch.len, h = ch.h0 - 1;
+  const int tot = n * (n + 1) / 2;
+  int32_t* Pt = lds;           // row q at tri_row_off(q), columns from 4 floor(q / 4)
+  int32_t* Bt = lds + RW;
+  int32_t* M = lds + 2 * RW;   // [S][S + 1]
+  __shared__ int64_t rowo[S];  // row offsets of the chain's nodes
+  for (int q = tid; q < 2 * RW + S * LD; q += NT) lds[q] = INF32;
+  if (tid < n) rowo[tid] = a.dis_hash[c.tnode[ch.first + tid]];
+  __syncthreads();
+  {
+    const uint64_t pb = (uint64_t)c.pro + ch.tri, tb = (uint64_t)c.tro + ch.tri;
+    for (int q0 = 0; q0 < tot; q0 += 4 * NT) {
+      int32_t xp[4], xt[4];
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int q = min(q0 + r * NT + tid, tot - 1);
+        xp[r] = ld<IDX32>(a.dis, pb + q);
+        xt[r] = ld<IDX32>(a.dis, tb + q);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int q = q0 + r * NT + tid;
+        if (q < tot) {
+          const int i = tri_row(q), m = q - i * (i + 1) / 2;  // P[i][m], t'_m(i)
+          const int ro = tri_row_off(S, m) - (m & ~3);
+          Pt[ro + i] = xp[r];
+          if (m < i) Bt[ro + i] = min(xp[r], xt[r]);
+          else M[i * LD + i] = 0;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // work item -> (mb, c, ib): groups (mb, c) of NB - mb items each
+  int j = tid, mb = 0, cs = 0, ib = -1;
+  for (int x = 0; x < NB; x++) {
+    const int nc = (4 * x + 4 + TRI_CQ - 1) / TRI_CQ;
+    if (j < nc * (NB - x)) {
+      mb = x;
+      cs = j / (NB - x);
+      ib = x + j % (NB - x);
+      break;
+    }
+    j -= nc * (NB - x);
+  }
+  const int q0 = cs * TRI_CQ, q1 = min(min(q0 + TRI_CQ, 4 * mb + 4), n);
+  if (ib >= 0 && 4 * ib < n && q0 < q1) {
+    int32_t cv[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) cv[x][y] = INF32;
+    for (int q = q0; q < q1; q++) {
+      const int ro = tri_row_off(S, q) - (q & ~3);
+      const int4 bv = *(const int4*)(Bt + ro + 4 * ib);
+      const int4 pv = *(const int4*)(Pt + ro + 4 * mb);
+      const int32_t bb[4] = {bv.x, bv.y, bv.z, bv.w}, pp[4] = {pv.x, pv.y, pv.z, pv.w};
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) cv[x][y] = min(cv[x][y], __builtin_elementwise_add_sat(bb[x], pp[y]));
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const int i = 4 * ib + x, m = 4 * mb + y;
+        if (i < n && m < i) atomicMin(&M[i * LD + m], cv[x][y]);
+      }
+  }
+  __syncthreads();
+  // rows: cell q of the row-major triangle -> (i, m); transposed rows: cell
+  // q of the triangle taken from the last column -> (m, i), lanes on
+  // consecutive i
+  for (int q = tid; q < tot; q += NT) {
+    const int i = tri_row(q), m = q - i * (i + 1) / 2;
+    a.dis[rowo[i] + h + m] = M[i * LD + m];
+  }
+  for (int q = tid; q < tot; q += NT) {
+    const int r = tri_row(q), s2 = q - r * (r + 1) / 2;
+    const int m = n - 1 - r, i = n - 1 - s2;
+    a.dis[(uint64_t)a.tbo + (uint64_t)(h + m) * (uint64_t)a.nt + ch.first + i] = M[i * LD + m];
+  }
+}
+
+template <int S>
+void launch_tri_s(const ChainArgs& c, int nunits, hipStream_t s) {
+  if (c.a.idx32) hipLaunchKernelGGL((chain_tri_kernel<S, true>), xcd_grid(nunits), dim3(fw_threads(S)), tri_lds(S), s, c);
+  else hipLaunchKernelGGL((chain_tri_kernel<S, false>), xcd_grid(nunits), dim3(fw_threads(S)), tri_lds(S), s, c);
+}
+
 // Dynamic LDS above 64 KB has to be requested per kernel, for exactly what
 // its largest launch uses (static LDS counts against the same 160 KB).
 template <typename K>
@@ -752,13 +890,26 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
     c.begin = rd.k3b;
     c.count = rd.k3n;
     const dim3 grid = xcd_grid(rd.k3n);
-    if (i32) hipLaunchKernelGGL((chain_cross_kernel<true>), grid, dim3(256), 0, s, c);
-    else hipLaunchKernelGGL((chain_cross_kernel<false>), grid, dim3(256), 0, s, c);
+    if (c.tri) {
+      if (i32) hipLaunchKernelGGL((chain_cross_kernel<true, true>), grid, dim3(256), 0, s, c);
+      else hipLaunchKernelGGL((chain_cross_kernel<false, true>), grid, dim3(256), 0, s, c);
+    } else {
+      if (i32) hipLaunchKernelGGL((chain_cross_kernel<true, false>), grid, dim3(256), 0, s, c);
+      else hipLaunchKernelGGL((chain_cross_kernel<false, false>), grid, dim3(256), 0, s, c);
+    }
   } else if (step == 3 && rd.k4n > 0) {
     c.begin = rd.k4b;
     c.count = rd.k4n;
-    if (l4) launch_fw<FW_L4>(c, rd.k4n, rd.lmax, s);
-    else launch_fw<FW_L3>(c, rd.k4n, rd.lmax, s);
+    if (l4 && c.tri) {
+      const int S = fw_size(rd.lmax);
+      if (S == 32) launch_tri_s<32>(c, rd.k4n, s);
+      else if (S == 64) launch_tri_s<64>(c, rd.k4n, s);
+      else launch_tri_s<128>(c, rd.k4n, s);
+    } else if (l4) {
+      launch_fw<FW_L4>(c, rd.k4n, rd.lmax, s);
+    } else {
+      launch_fw<FW_L3>(c, rd.k4n, rd.lmax, s);
+    }
   }
 }
 
@@ -781,6 +932,12 @@ hipError_t set_chain_lds_limits() {
   if (e == hipSuccess) e = set_fw_limits<FW_CLOSURE>();
   if (e == hipSuccess) e = set_fw_limits<FW_L3>();
   if (e == hipSuccess) e = set_fw_limits<FW_L4>();
+  for (auto x : {(const void*)chain_tri_kernel<32, true>, (const void*)chain_tri_kernel<32, false>})
+    if (e == hipSuccess) e = hipFuncSetAttribute(x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tri_lds(32));
+  for (auto x : {(const void*)chain_tri_kernel<64, true>, (const void*)chain_tri_kernel<64, false>})
+    if (e == hipSuccess) e = hipFuncSetAttribute(x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tri_lds(64));
+  for (auto x : {(const void*)chain_tri_kernel<128, true>, (const void*)chain_tri_kernel<128, false>})
+    if (e == hipSuccess) e = hipFuncSetAttribute(x, hipFuncAttributeMaxDynamicSharedMemorySize, (int)tri_lds(128));
   return e;
 }
 // ---- subtree tasks (BATCHED bottom part) ----
@@ -1301,11 +1458,20 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     P.rounds.push_back(rd);
     c0 = c1;
   }
-  P.kcl_b = (int)units.size();  // every chain, for the closure launch
-  for (size_t ci = 0; ci < corder.size(); ci++) units.push_back(make_int4((int)ci, 0, 0, 0));
-  P.kcl_n = (int)corder.size();
-  P.kcl_lmax = 0;
-  for (const auto& x : ch) P.kcl_lmax = std::max(P.kcl_lmax, (int)x.len);
+  // the closure of every chain: one launch per size class (the largest
+  // first: it takes longest), chains in chain-depth order inside each
+  for (int sc = 0; sc < 3; sc++) {
+    const int S = 128 >> sc;
+    P.kcl_b[sc] = (int)units.size();
+    for (size_t ci = 0; ci < corder.size(); ci++)
+      if (fw_size(ch[ci].len) == S) units.push_back(make_int4((int)ci, 0, 0, 0));
+    P.kcl_n[sc] = (int)units.size() - P.kcl_b[sc];
+  }
+  // the task closures start once the chain depths whose triangles need a
+  // whole CU's LDS are past (they would hold those CUs' LDS meanwhile)
+  P.tclose_after = 0;
+  for (size_t r = 0; r < P.rounds.size(); r++)
+    if (fw_size(P.rounds[r].lmax) == 128) P.tclose_after = (int)r + 1;
   H2H_TRY(upload(ctx, P.chains, ch.data(), ch.size() * sizeof(Chain)));
   H2H_TRY(upload(ctx, P.tnode, tnode.data(), tnode.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.touter, touter4.data(), touter4.size() * sizeof(int32_t)));  // read by the L4 cross only
@@ -1522,6 +1688,7 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     c.tro = P.tro;
     c.pro = P.pro;
     c.begin = c.count = 0;
+    c.tri = env_int("H2H_TRI_PRODUCT", 1, 0, 1) != 0 ? 1 : 0;
     TaskArgs t;
     t.a = a;
     t.lvl_base = P.task_lvl_base.as<int32_t>();
@@ -1547,30 +1714,48 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     // The two closures depend only on the tree's bag weights: off the timed
     // path (ev == nullptr) they run on a side stream beside the chain rounds
     // and are joined where their results are first read -- the chains'
-    // before the first outer-column apply, the tasks' before the task passes.
+    // before the first launch that reads P, the tasks' before the task passes.
     hipStream_t s2 = ev ? nullptr : ctx->aux_stream;
     if (s2) {
       H2H_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
       H2H_HIP(ctx, hipStreamWaitEvent(s2, ctx->ev_fork, 0));
-      if (P.kcl_n > 0) launch_chain_closure(c, P.kcl_b, P.kcl_n, P.kcl_lmax, s2);
+      for (int sc = 0; sc < 3; sc++)
+        if (P.kcl_n[sc] > 0) launch_chain_closure(c, P.kcl_b[sc], P.kcl_n[sc], 128 >> sc, s2);
       H2H_HIP(ctx, hipEventRecord(ctx->ev_join1, s2));
+    } else {
+      for (int sc = 0; sc < 3; sc++)
+        if (P.kcl_n[sc] > 0) H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b[sc], P.kcl_n[sc], 128 >> sc, s); }));
+    }
+    // task closures on the side stream, after chain depth tclose_after
+    auto fork_task_close = [&]() -> h2h_status {
+      if (!s2) return H2H_OK;
+      H2H_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
+      H2H_HIP(ctx, hipStreamWaitEvent(s2, ctx->ev_fork, 0));
       if (task_close) launch_tasks(t, s2, 0);
       H2H_HIP(ctx, hipEventRecord(ctx->ev_join2, s2));
-    } else if (P.kcl_n > 0) {
-      H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b, P.kcl_n, P.kcl_lmax, s); }));
-    }
+      return H2H_OK;
+    };
+    if (P.tclose_after == 0) H2H_TRY(fork_task_close());
     bool joined1 = s2 == nullptr;
-    for (const auto& rd : P.rounds) {
+    for (size_t r = 0; r < P.rounds.size(); r++) {
+      const auto& rd = P.rounds[r];
+      if ((int)r == P.tclose_after && r > 0) H2H_TRY(fork_task_close());
       const int n[4] = {rd.k1n, rd.k2n, a.l4 ? rd.k3n : 0, rd.k4n};
       for (int step = 0; step < 4; step++) {
         if (n[step] == 0) continue;
+        // the closures P are read by the outer-column apply and by the L4
+        // triangle product; chain depths before the first apply (the root
+        // chain) take the Floyd-Warshall triangle instead of waiting
         if (step == 1 && !joined1) {
           H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join1, 0));
           joined1 = true;
         }
-        H2H_TRY(timed([&] { launch_chain_round(c, rd, s, step); }));
+        ChainArgs cs = c;
+        if ((step == 2 || step == 3) && !joined1) cs.tri = 0;
+        H2H_TRY(timed([&] { launch_chain_round(cs, rd, s, step); }));
       }
     }
+    if (P.tclose_after >= (int)P.rounds.size() && P.tclose_after > 0) H2H_TRY(fork_task_close());
     if (!joined1) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join1, 0));
     if (s2) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join2, 0));
     if (P.ntasks > 0) {
