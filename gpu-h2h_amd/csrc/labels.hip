@@ -975,7 +975,8 @@ struct TaskArgs {
   const int32_t* ie_w;
   int32_t* cl_w;             // closure weights: node k at banc[k] + k, d + 1 of them (self last)
   const int64_t* cl_ro;      // their rows
-  const int4* units;         // task_apply_kernel: (BFS node, first column, outer columns, entries)
+  const int4* units;         // task_apply_kernel: (BFS node, first column, outer columns | entries << 25, first csel)
+  const int32_t* csel;       // closure entries read, per BFS node (ancestors with entries above the task)
   int ubegin, nunits;        // units of this launch
   int ntasks;
 };
@@ -1020,8 +1021,7 @@ __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   if (u >= t.nunits) return;  // whole wavefronts
   const LabelArgs& a = t.a;
   const int4 un = t.units[t.ubegin + u];
-  const int k = un.x, c0 = un.y, L = un.z, ne = un.w;
-  const int64_t e0 = t.banc[k] + k;
+  const int k = un.x, c0 = un.y, L = un.z & ((1 << 25) - 1), ne = un.z >> 25;
   const int64_t rowv = a.dis_hash[t.bnodes[k]];
   int32_t acc[R];
 #pragma unroll
@@ -1029,8 +1029,9 @@ __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   int mw = 0;
   int64_t mro = 0;
   if (lane < ne) {  // at most TASK_H entries
-    mw = t.cl_w[e0 + lane];
-    mro = t.cl_ro[e0 + lane];
+    const int e = t.csel[un.w + lane];
+    mw = t.cl_w[e];
+    mro = t.cl_ro[e];
   }
   int q = 0;
   for (; q + 1 < ne; q += 2) {
@@ -1293,6 +1294,8 @@ h2h_status plan_items(h2h_ctx* ctx, int method, LabelPlan& P) {
 
 h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   const int n = ctx->n;
+  if (ctx->tree_height >= (1 << 25))
+    return set_err(ctx, H2H_ERR_INPUT, "batched plan: tree deeper than 2^25 levels");
   // maximal subtrees of at most smax nodes and TASK_H levels become tasks
   std::vector<int32_t> task_of(n, -1);
   const int64_t smax = env_int("H2H_TASK_NODES",
@@ -1493,6 +1496,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     std::vector<uint16_t> tanl;
     std::vector<uint8_t> ie_s;
     std::vector<int4> aunits, tinfo;
+    std::vector<int32_t> csel;  // closure entries task_apply_kernel reads (indices into cl_w / cl_ro)
     std::vector<int32_t> local(n, -1);
     P.tbound_b = (int)bunits.size();
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
@@ -1515,6 +1519,9 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
       std::vector<BoundUnit> tb_units;
       std::vector<int4> ta_units;
+      // members with entries above the task: only their bounds are finite,
+      // so only they get bound units and closure entries
+      std::vector<uint8_t> has_outer(mem.size(), 0);
       for (size_t k = 0; k < mem.size(); k++) {
         const int v = mem[k];
         const int d = ctx->depth[v] - r;
@@ -1525,6 +1532,23 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           nouter++;
           nouter4 += prune && ctx->pos_dom[e] ? 0 : 1;
         }
+        has_outer[k] = nouter > 0;
+        // closure entries of v: its in-task ancestors (task depth c < d) and
+        // itself (c = d) that have entries above the task
+        const int sel0 = (int)csel.size();
+        {
+          int x = v;
+          std::vector<int> anc_local(d + 1);
+          anc_local[d] = (int)k;
+          for (int q = d - 1; q >= 0; q--) {
+            x = ctx->father[x];
+            anc_local[q] = local[x];
+          }
+          const int base = (int)tanc.size() + kg;  // banc[kg] + kg
+          for (int c = 0; c <= d; c++)
+            if (has_outer[anc_local[c]]) csel.push_back(base + c);
+        }
+        const int nsel = (int)csel.size() - sel0;
         for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
           BoundUnit b;
           b.v = v;
@@ -1535,8 +1559,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           b.cfirst = fc.first;
           b.ch0 = fc.h0;
           b.nouter4 = nouter4;
-          tb_units.push_back(b);
-          ta_units.push_back(make_int4(kg, q0, r - 1, d + 1));
+          if (nouter > 0) tb_units.push_back(b);
+          ta_units.push_back(make_int4(kg, q0, (r - 1) | (nsel << 25), sel0));
         }
         if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
           starts.push_back(kg);
@@ -1595,6 +1619,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_ie_s, ie_s.data(), ie_s.size()));
     H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_cl_ro, cl_ro.data(), cl_ro.size() * sizeof(int64_t)));
+    H2H_TRY(upload(ctx, P.task_csel, csel.data(), std::max<size_t>(csel.size(), 1) * sizeof(int32_t)));
     H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_ro.size(), 1) * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_units, aunits.data(), aunits.size() * sizeof(int4)));
     P.ntasks = ntasks;
@@ -1706,6 +1731,7 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.ie_w = P.task_ie_w.as<int32_t>();
     t.cl_w = P.task_cl_w.as<int32_t>();
     t.cl_ro = P.task_cl_ro.as<int64_t>();
+    t.csel = P.task_csel.as<int32_t>();
     t.units = P.task_units.as<int4>();
     t.ubegin = 0;
     t.nunits = P.task_nunits;
