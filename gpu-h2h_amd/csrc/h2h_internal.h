@@ -75,7 +75,8 @@ struct LabelPlan {
   DevArray task_lvl_base, task_lvl_off, task_lvl_depth, task_nodes;  // subtree tasks: BFS levels
   DevArray task_tbase, task_troot, task_banc, task_tanc, task_tanl, task_bn_ie, task_ie_s, task_ie_w;
   DevArray task_cl_w, task_cl_ro, task_units, bunits;  // closure lists; bound units of chains and tasks
-  DevArray task_csel;                                  // closure entries the task apply reads
+  DevArray task_cl_dst;                                // where the task closure stores each entry
+  int task_ra = 8;                                     // task apply: columns per lane
   DevArray task_tinfo;
   DevArray bpos_pos, bpos_w, bpos_rowoff;  // batched plan's entry copy: kept outer entries, dropped ones, the rest
   bool pruned = false;

@@ -746,7 +746,7 @@ template <int S, bool IDX32>
 __global__ void __launch_bounds__(fw_threads(S)) chain_tri_kernel(ChainArgs c) {
   constexpr int NT = fw_threads(S), LD = S + 1, NB = S / 4, RW = tri_rows_words(S);
   extern __shared__ int32_t lds[];
-  const int tid = threadIdx.x, lane = tid & 63;
+  const int tid = threadIdx.x;
   const int u = xcd_index(c.count);
   if (u >= c.count) return;  // whole workgroup
   const LabelArgs& a = c.a;
@@ -973,12 +973,14 @@ struct TaskArgs {
   const int32_t* bn_ie;      // per BFS node: first inner entry (BFS node count + 1 offsets)
   const uint8_t* ie_s;       // inner entries: task depth of the bag member
   const int32_t* ie_w;
-  int32_t* cl_w;             // closure weights: node k at banc[k] + k, d + 1 of them (self last)
+  const int32_t* cl_dst;     // per BFS node k and task depth c (at banc[k] + k + c, self last): where
+                             // task_close_kernel stores P[k][c] in cl_w, -1 when the apply skips it
+  int32_t* cl_w;             // closure entries the apply reads, node by node (ancestors with entries above the task)
   const int64_t* cl_ro;      // their rows
-  const int4* units;         // task_apply_kernel: (BFS node, first column, outer columns | entries << 25, first csel)
-  const int32_t* csel;       // closure entries read, per BFS node (ancestors with entries above the task)
+  const int4* units;         // task_apply_kernel: (BFS node, first column, outer columns | entries << 25, first cl_w)
   int ubegin, nunits;        // units of this launch
   int ntasks;
+  int ra;                    // task_apply_kernel columns per lane (units of 64 ra columns)
 };
 
 // P[v][c] = min over the in-task entries (s, w) of v with s >= c of
@@ -1005,7 +1007,8 @@ __global__ void __launch_bounds__(256) task_close_kernel(TaskArgs t) {
         }
       }
       Pl[(k - base) * TASK_H + c] = x;
-      t.cl_w[t.banc[k] + k + c] = x;
+      const int dst = t.cl_dst[t.banc[k] + k + c];
+      if (dst >= 0) t.cl_w[dst] = x;
     }
     __syncthreads();
   }
@@ -1014,7 +1017,7 @@ __global__ void __launch_bounds__(256) task_close_kernel(TaskArgs t) {
 // Outer columns of task nodes: one wavefront per (node, 512 columns),
 // min over the closure entries (P, row) of w + dis[row][j], two entries per
 // round of loads.
-template <bool IDX32>
+template <bool IDX32, int RA>
 __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int u = xcd_index((t.nunits + 3) >> 2) * 4 + wid;
@@ -1023,40 +1026,39 @@ __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   const int4 un = t.units[t.ubegin + u];
   const int k = un.x, c0 = un.y, L = un.z & ((1 << 25) - 1), ne = un.z >> 25;
   const int64_t rowv = a.dis_hash[t.bnodes[k]];
-  int32_t acc[R];
+  int32_t acc[RA];
 #pragma unroll
-  for (int r = 0; r < R; r++) acc[r] = INF32;
+  for (int r = 0; r < RA; r++) acc[r] = INF32;
   int mw = 0;
   int64_t mro = 0;
   if (lane < ne) {  // at most TASK_H entries
-    const int e = t.csel[un.w + lane];
-    mw = t.cl_w[e];
-    mro = t.cl_ro[e];
+    mw = t.cl_w[un.w + lane];
+    mro = t.cl_ro[un.w + lane];
   }
   int q = 0;
   for (; q + 1 < ne; q += 2) {
     const int w0 = __builtin_amdgcn_readlane(mw, q), w1 = __builtin_amdgcn_readlane(mw, q + 1);
     const uint64_t u0 = (uint64_t)readlane64(mro, q), u1 = (uint64_t)readlane64(mro, q + 1);
-    int32_t v0[R], v1[R];
+    int32_t v0[RA], v1[RA];
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+    for (int r = 0; r < RA; r++) {
       const int j = min(c0 + r * 64 + lane, L - 1);
       v0[r] = ld<IDX32>(a.dis, u0 + j);
       v1[r] = ld<IDX32>(a.dis, u1 + j);
     }
 #pragma unroll
-    for (int r = 0; r < R; r++)  // L3 bounds can be INF: saturating adds
+    for (int r = 0; r < RA; r++)  // L3 bounds can be INF: saturating adds
       acc[r] = min(acc[r], min(__builtin_elementwise_add_sat(w0, v0[r]), __builtin_elementwise_add_sat(w1, v1[r])));
   }
   if (q < ne) {
     const int w0 = __builtin_amdgcn_readlane(mw, q);
     const uint64_t u0 = (uint64_t)readlane64(mro, q);
 #pragma unroll
-    for (int r = 0; r < R; r++)
+    for (int r = 0; r < RA; r++)
       acc[r] = min(acc[r], __builtin_elementwise_add_sat(w0, ld<IDX32>(a.dis, u0 + min(c0 + r * 64 + lane, L - 1))));
   }
 #pragma unroll
-  for (int r = 0; r < R; r++) {
+  for (int r = 0; r < RA; r++) {
     const int j = c0 + r * 64 + lane;
     if (j < L) a.dis[rowv + j] = acc[r];
   }
@@ -1212,8 +1214,13 @@ void launch_tasks(const TaskArgs& t, hipStream_t s, int step) {
     hipLaunchKernelGGL(task_close_kernel, xcd_grid(t.ntasks), dim3(256), 0, s, t);
   } else if (step == 1) {
     const dim3 grid = xcd_grid((t.nunits + 3) / 4);
-    if (i32) hipLaunchKernelGGL((task_apply_kernel<true>), grid, dim3(256), 0, s, t);
-    else hipLaunchKernelGGL((task_apply_kernel<false>), grid, dim3(256), 0, s, t);
+    if (t.ra == 16) {
+      if (i32) hipLaunchKernelGGL((task_apply_kernel<true, 16>), grid, dim3(256), 0, s, t);
+      else hipLaunchKernelGGL((task_apply_kernel<false, 16>), grid, dim3(256), 0, s, t);
+    } else {
+      if (i32) hipLaunchKernelGGL((task_apply_kernel<true, R>), grid, dim3(256), 0, s, t);
+      else hipLaunchKernelGGL((task_apply_kernel<false, R>), grid, dim3(256), 0, s, t);
+    }
   } else {
     const dim3 grid = xcd_grid(t.ntasks);
     if (i32) {
@@ -1492,11 +1499,14 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     for (int i = 0; i < n; i++)
       if (task_of[ctx->bfs[i]] >= 0) members[task_of[ctx->bfs[i]]].push_back(ctx->bfs[i]);
     std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth, bnodes, banc, tbase, troot, bn_ie(1, 0), ie_w;
-    std::vector<int64_t> tanc, cl_ro;
+    std::vector<int64_t> tanc;
     std::vector<uint16_t> tanl;
     std::vector<uint8_t> ie_s;
     std::vector<int4> aunits, tinfo;
-    std::vector<int32_t> csel;  // closure entries task_apply_kernel reads (indices into cl_w / cl_ro)
+    std::vector<int32_t> cl_dst;  // per (node, task depth): index in the apply's closure entries, or -1
+    std::vector<int64_t> cl_rc;   // the apply's closure entries: rows
+    const int ra = (int)env_int("H2H_TASK_RA", 8, 8, 16) == 16 ? 16 : 8;
+    P.task_ra = ra;
     std::vector<int32_t> local(n, -1);
     P.tbound_b = (int)bunits.size();
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
@@ -1535,20 +1545,24 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         has_outer[k] = nouter > 0;
         // closure entries of v: its in-task ancestors (task depth c < d) and
         // itself (c = d) that have entries above the task
-        const int sel0 = (int)csel.size();
+        const int sel0 = (int)cl_rc.size();
         {
           int x = v;
-          std::vector<int> anc_local(d + 1);
+          std::vector<int> anc_local(d + 1), anc_v(d + 1);
           anc_local[d] = (int)k;
+          anc_v[d] = v;
           for (int q = d - 1; q >= 0; q--) {
             x = ctx->father[x];
             anc_local[q] = local[x];
+            anc_v[q] = x;
           }
-          const int base = (int)tanc.size() + kg;  // banc[kg] + kg
-          for (int c = 0; c <= d; c++)
-            if (has_outer[anc_local[c]]) csel.push_back(base + c);
+          for (int c = 0; c <= d; c++) {
+            const bool keep = has_outer[anc_local[c]];
+            cl_dst.push_back(keep ? (int32_t)cl_rc.size() : -1);  // at banc[kg] + kg + c
+            if (keep) cl_rc.push_back(ctx->dis_hash[anc_v[c]]);
+          }
         }
-        const int nsel = (int)csel.size() - sel0;
+        const int nsel = (int)cl_rc.size() - sel0;
         for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
           BoundUnit b;
           b.v = v;
@@ -1560,8 +1574,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           b.ch0 = fc.h0;
           b.nouter4 = nouter4;
           if (nouter > 0) tb_units.push_back(b);
-          ta_units.push_back(make_int4(kg, q0, (r - 1) | (nsel << 25), sel0));
         }
+        for (int q0 = 0; q0 < r - 1; q0 += 64 * ra) ta_units.push_back(make_int4(kg, q0, (r - 1) | (nsel << 25), sel0));
         if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
           starts.push_back(kg);
           lvl_depth.push_back(ctx->depth[v]);
@@ -1577,8 +1591,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           tanc[at + q] = ctx->dis_hash[x];
           tanl[at + q] = (uint16_t)local[x];
         }
-        for (int q = 0; q < d; q++) cl_ro.push_back(tanc[at + q]);
-        cl_ro.push_back(ctx->dis_hash[v]);
+
         for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++) {
           const int pp = ctx->pos_pos[e];
           if (pp >= r - 1) {
@@ -1596,7 +1609,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         for (const auto& b : tb_units)
           if (b.c0 == q0) bunits.push_back(b);
         for (const auto& x : ta_units)
-          if (x.y == q0) aunits.push_back(x);
+          if (x.y == q0) aunits.push_back(x);  // (apply units of 64 ra columns start on these too)
       }
       for (int v : mem) local[v] = -1;
     }
@@ -1618,9 +1631,9 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_bn_ie, bn_ie.data(), bn_ie.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_ie_s, ie_s.data(), ie_s.size()));
     H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_cl_ro, cl_ro.data(), cl_ro.size() * sizeof(int64_t)));
-    H2H_TRY(upload(ctx, P.task_csel, csel.data(), std::max<size_t>(csel.size(), 1) * sizeof(int32_t)));
-    H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_ro.size(), 1) * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, P.task_cl_ro, cl_rc.data(), std::max<size_t>(cl_rc.size(), 1) * sizeof(int64_t)));
+    H2H_TRY(upload(ctx, P.task_cl_dst, cl_dst.data(), std::max<size_t>(cl_dst.size(), 1) * sizeof(int32_t)));
+    H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_rc.size(), 1) * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_units, aunits.data(), aunits.size() * sizeof(int4)));
     P.ntasks = ntasks;
     P.task_nodes_n = (int64_t)bnodes.size();
@@ -1731,7 +1744,8 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.ie_w = P.task_ie_w.as<int32_t>();
     t.cl_w = P.task_cl_w.as<int32_t>();
     t.cl_ro = P.task_cl_ro.as<int64_t>();
-    t.csel = P.task_csel.as<int32_t>();
+    t.cl_dst = P.task_cl_dst.as<int32_t>();
+    t.ra = P.task_ra;
     t.units = P.task_units.as<int4>();
     t.ubegin = 0;
     t.nunits = P.task_nunits;
