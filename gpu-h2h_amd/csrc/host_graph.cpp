@@ -7,6 +7,8 @@
 #include <cstring>
 #include <numeric>
 #include <queue>
+#include <thread>
+#include <tuple>
 
 #include "h2h_internal.h"
 
@@ -295,25 +297,36 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
   const bool dom = ctx->bag_dom.size() == ctx->bag_nbr.size() && !ctx->bag_dom.empty();
   if (dom) ctx->pos_dom.assign(P, 0);
   else ctx->pos_dom.clear();
-  std::vector<std::tuple<int32_t, int32_t, int32_t, uint8_t>> ent;
-  for (int v = 0; v < n; v++) {
-    ent.clear();
-    for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
-      int32_t u = ctx->bag_nbr[e];
-      ent.emplace_back(ctx->depth[u] - 1, u, ctx->bag_w[e], dom ? ctx->bag_dom[e] : 0);
+  // each node's bag sorted by position; nodes are independent, so the host
+  // threads take contiguous node ranges
+  auto layout = [&](int v0, int v1) {
+    std::vector<std::tuple<int32_t, int32_t, int32_t, uint8_t>> ent;
+    for (int v = v0; v < v1; v++) {
+      ent.clear();
+      for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
+        int32_t u = ctx->bag_nbr[e];
+        ent.emplace_back(ctx->depth[u] - 1, u, ctx->bag_w[e], dom ? ctx->bag_dom[e] : 0);
+      }
+      std::sort(ent.begin(), ent.end());
+      int64_t k = ctx->pos_hash[v];
+      for (auto& t : ent) {
+        ctx->pos_pos[k] = std::get<0>(t);
+        ctx->pos_id[k] = std::get<1>(t);
+        ctx->pos_w[k] = std::get<2>(t);
+        if (dom) ctx->pos_dom[k] = std::get<3>(t);
+        k++;
+      }
+      ctx->pos_pos[k] = ctx->depth[v] - 1;
+      ctx->pos_id[k] = v;
+      ctx->pos_w[k] = 0;
     }
-    std::sort(ent.begin(), ent.end());
-    int64_t k = ctx->pos_hash[v];
-    for (auto& t : ent) {
-      ctx->pos_pos[k] = std::get<0>(t);
-      ctx->pos_id[k] = std::get<1>(t);
-      ctx->pos_w[k] = std::get<2>(t);
-      if (dom) ctx->pos_dom[k] = std::get<3>(t);
-      k++;
-    }
-    ctx->pos_pos[k] = ctx->depth[v] - 1;
-    ctx->pos_id[k] = v;
-    ctx->pos_w[k] = 0;
+  };
+  {
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < T; i++) pool.emplace_back(layout, (int)((int64_t)n * i / T), (int)((int64_t)n * (i + 1) / T));
+    layout(0, (int)((int64_t)n / T));
+    for (auto& th : pool) th.join();
   }
   // BFS levels (TreeBFS / TreeHash)
   ctx->bfs.clear();
