@@ -385,11 +385,7 @@ h2h_status h2h_query_device(h2h_ctx* ctx, int kind, int exec, const int32_t* d_x
   if (n < 0 || (n > 0 && (!d_x || !d_y || !d_out))) return set_err(ctx, H2H_ERR_ARG, "bad query buffers");
   if (n == 0) return H2H_OK;
   hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-  if (exec == H2H_EXEC_BATCHED) {
-    h2h_status st = query_workspace(ctx, ctx->qws, n);
-    if (st) return st;
-  }
-  return launch_query(ctx, kind, exec, d_x, d_y, d_out, n, s, &ctx->qws);
+  return launch_query(ctx, kind, exec, d_x, d_y, d_out, n, s);
 }
 
 h2h_status h2h_query(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const int32_t* y, int32_t* out,

@@ -57,12 +57,6 @@ struct LevelPlan {
   int max_np = 0;
 };
 
-// Device scratch of one batched query launch (one per stream that runs them).
-struct QueryWorkspace {
-  DevArray lca, cls, order, ctr;
-  int64_t cap = 0;
-};
-
 // Launch plan of one label-build method over the current tree: work items
 // grouped by level, the nodes whose rows start at INF, and (for the batched
 // method) the captured launch sequence per label kind.
@@ -178,7 +172,6 @@ struct h2h_ctx {
   h2h::DevArray d_depth, d_bfs, d_father, d_anc, d_lca, d_tin;
   h2h::DevArray d_qx, d_qy, d_qout;   // staging for host-buffer queries
   size_t q_cap = 0;
-  h2h::QueryWorkspace qws;
   h2h::LabelPlan lplan[4];            // indexed by H2H_METHOD_*
   int lca_levels = 0;
   int num_cus = 256;
@@ -221,7 +214,6 @@ void invalidate_label_plans(h2h_ctx* ctx);                             // labels
 h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kernel_us, int64_t* launches);
 h2h_status build_lca(h2h_ctx* ctx);                                    // query.hip
 h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, const int32_t* dy,
-                        int32_t* dout, int64_t n, hipStream_t s, QueryWorkspace* ws);  // query.hip
-h2h_status query_workspace(h2h_ctx* ctx, QueryWorkspace& ws, int64_t n);             // query.hip
+                        int32_t* dout, int64_t n, hipStream_t s);  // query.hip
 
 }  // namespace h2h

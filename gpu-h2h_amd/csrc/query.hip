@@ -14,10 +14,11 @@
 // Execution
 //   plain    one lane per query, like the reference kernels;
 //   (an id outside [0, n) answers -1 instead of reading out of bounds)
-//   batched  queries are first sized (LCA and its width), then split into
-//            width classes that run with 4, 16 or 64 lanes per query: a
-//            query's label reads become coalesced runs and the lanes of a
-//            wavefront finish together.
+//   batched  one launch: each workgroup sizes its tile of queries (LCA and
+//            its width), groups them by width class in LDS and runs the
+//            classes with 64, 16 or 4 lanes per query: a query's label
+//            reads become coalesced runs and the lanes of a wavefront
+//            finish together.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,79 +93,39 @@ __global__ void __launch_bounds__(256) query_plain_kernel(QArgs q, const int32_t
 }
 
 // ---- batched ----
-// pass 1: LCA, the width of the scan, and the width class of every query
-__global__ void __launch_bounds__(256) query_size_kernel(QArgs q, const int32_t* __restrict__ x,
-                                                       const int32_t* __restrict__ y, int64_t nq, int kind,
-                                                       int32_t* __restrict__ lca_out, int32_t* __restrict__ cls_out,
-                                                       int32_t* __restrict__ cls_count) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int c = -1;
-  if (i < nq) {
-    const int a = x[i], b = y[i];
-    int node = -1, width = 0;
-    if ((unsigned)a >= (unsigned)q.n || (unsigned)b >= (unsigned)q.n) {
-      node = -2;  // id out of range
-    } else if (a != b) {
-      const Lca L = find_lca(q, a, b);
-      node = L.node;
-      width = (kind == H2H_QUERY_LCA) ? (int)(q.pos_hash[L.node + 1] - q.pos_hash[L.node]) : L.depth;
-    }
-    c = width <= 8 ? 0 : (width <= 48 ? 1 : 2);
-    lca_out[i] = node;
-  }
-  // per-wave aggregation of the class counters
-  for (int k = 0; k < 3; k++) {
-    const uint64_t m = __ballot(c == k);
-    if (m && (threadIdx.x & 63) == __ffsll((unsigned long long)m) - 1) atomicAdd(&cls_count[k], __popcll(m));
-  }
-  if (i < nq) cls_out[i] = c;
-}
 
-// pass 2: scatter query indices into their class lists
-__global__ void __launch_bounds__(256) query_scatter_kernel(const int32_t* __restrict__ cls, int64_t nq,
-                                                          const int32_t* __restrict__ cls_base,
-                                                          int32_t* __restrict__ cls_fill, int32_t* __restrict__ order) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = (i < nq) ? cls[i] : -1;
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  for (int k = 0; k < 3; k++) {
-    const uint64_t m = __ballot(c == k);
-    if (!m) continue;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int basepos = 0;
-    if (lane == leader) basepos = atomicAdd(&cls_fill[k], __popcll(m));
-    basepos = __shfl(basepos, leader, 64);
-    if (c == k) order[cls_base[k] + basepos + __popcll(m & lt)] = (int32_t)i;
-  }
-}
+// Batched queries in one launch: a workgroup takes up to 256 queries (fewer
+// for small batches, so that they still spread over every CU), sizes them
+// (LCA, width, row offsets: one lane each, loads in parallel), groups them
+// by width class in LDS, and answers each class with 64, 16 or 4 lanes per
+// query.  The per-query metadata is in LDS when the label reads start, so
+// a query costs one dependent round of loads (noLCA) or two (LCA: the
+// positions, then the labels).
+struct QMeta {
+  int64_t rx, ry, e0;  // row offsets of x and y; first position entry of the LCA (LCA kind)
+  int32_t w, node;     // width (positions or common-ancestor columns); LCA, -1 for x == y, -2 out of range
+};
 
-// pass 3: one persistent launch over the three class lists, widest first;
-// class c runs G_c lanes per query, blocks stride over units of 256/G_c queries.
 template <int G>
-__device__ inline void group_pass(const QArgs& q, const int32_t* __restrict__ x, const int32_t* __restrict__ y,
-                                  int32_t* __restrict__ out, const int32_t* __restrict__ lca_node,
-                                  const int32_t* __restrict__ order, int64_t cnt, int kind) {
-  constexpr int PER = 256 / G;
+__device__ inline void tile_pass(const QArgs& q, int kind, const QMeta* __restrict__ meta, const int16_t* __restrict__ list,
+                                 int cnt, int64_t base, int32_t* __restrict__ out) {
+  constexpr int PER = 256 / G;  // queries per round of the workgroup
   const int g = threadIdx.x % G;
-  const int64_t units = (cnt + PER - 1) / PER;
-  for (int64_t u = blockIdx.x; u < units; u += gridDim.x) {
-    const int64_t slot = u * PER + threadIdx.x / G;
+  for (int s0 = 0; s0 < cnt; s0 += PER) {
+    const int slot = s0 + (int)threadIdx.x / G;
     int32_t r = INF32;
-    int64_t qi = -1;
+    int qi = -1;
     if (slot < cnt) {
-      qi = order[slot];
-      const int L = lca_node[qi];
-      if (L < 0) {
-        r = (L == -1) ? 0 : -1;
+      qi = list[slot];
+      const QMeta m = meta[qi];
+      if (m.node < 0) {
+        r = (m.node == -1) ? 0 : -1;
       } else {
-        const int32_t* dx = q.dis + q.dis_hash[x[qi]];
-        const int32_t* dy = q.dis + q.dis_hash[y[qi]];
-        // four strides per round: the position loads, then the eight label
-        // loads, are in flight together (clamped indices stay in the rows)
+        const int32_t* dx = q.dis + m.rx;
+        const int32_t* dy = q.dis + m.ry;
         if (kind == H2H_QUERY_LCA) {
-          const int64_t e0 = q.pos_hash[L], e1 = q.pos_hash[L + 1];
-          for (int64_t e = e0 + g; e < e1; e += 4 * G) {
+          const int64_t e1 = m.e0 + m.w;
+          for (int64_t e = m.e0 + g; e < e1; e += 4 * G) {
             int p[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) p[k] = q.pos_pos[min(e + k * G, e1 - 1)];
@@ -179,50 +140,78 @@ __device__ inline void group_pass(const QArgs& q, const int32_t* __restrict__ x,
               if (e + k * G < e1) r = min(r, a[k] + b[k]);
           }
         } else {
-          const int depthL = (int)(q.dis_hash[L + 1] - q.dis_hash[L]);
-          for (int j = g; j < depthL; j += 4 * G) {
+          for (int j = g; j < m.w; j += 4 * G) {
             int32_t a[4], b[4];
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-              const int jj = min(j + k * G, depthL - 1);
+              const int jj = min(j + k * G, m.w - 1);
               a[k] = dx[jj];
               b[k] = dy[jj];
             }
 #pragma unroll
             for (int k = 0; k < 4; k++)
-              if (j + k * G < depthL) r = min(r, a[k] + b[k]);
+              if (j + k * G < m.w) r = min(r, a[k] + b[k]);
           }
         }
       }
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) r = min(r, __shfl_xor(r, off, G));
-    if (qi >= 0 && g == 0) out[qi] = r;
+    if (qi >= 0 && g == 0) out[base + qi] = r;
   }
 }
 
-__global__ void __launch_bounds__(256) query_group_kernel(QArgs q, const int32_t* __restrict__ x,
-                                                        const int32_t* __restrict__ y, int32_t* __restrict__ out,
-                                                        const int32_t* __restrict__ lca_node,
-                                                        const int32_t* __restrict__ order,
-                                                        const int32_t* __restrict__ cls_count, int kind) {
-  const int64_t c0 = cls_count[0], c1 = cls_count[1], c2 = cls_count[2];
-  group_pass<64>(q, x, y, out, lca_node, order + c0 + c1, c2, kind);
-  group_pass<16>(q, x, y, out, lca_node, order + c0, c1, kind);
-  group_pass<4>(q, x, y, out, lca_node, order, c0, kind);
-}
-
-// class bookkeeping: zero the counters / turn counts into list bases
-__global__ void query_zero_kernel(int32_t* v, int n) {
-  if ((int)threadIdx.x < n) v[threadIdx.x] = 0;
-}
-__global__ void query_bases_kernel(const int32_t* cls_count, int32_t* cls_base, int32_t* cls_fill) {
-  if (threadIdx.x == 0) {
-    cls_base[0] = 0;
-    cls_base[1] = cls_count[0];
-    cls_base[2] = cls_count[0] + cls_count[1];
-    cls_fill[0] = cls_fill[1] = cls_fill[2] = 0;
+__global__ void __launch_bounds__(256) query_tile_kernel(QArgs q, const int32_t* __restrict__ x,
+                                                       const int32_t* __restrict__ y, int32_t* __restrict__ out,
+                                                       int64_t nq, int kind, int qt) {
+  __shared__ QMeta meta[256];
+  __shared__ int16_t list[3][256];
+  __shared__ int cnt[3];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int64_t base = (int64_t)blockIdx.x * qt;  // qt <= 256 queries per workgroup
+  const int64_t i = base + tid;
+  if (tid < 3) cnt[tid] = 0;
+  int c = -1;
+  if (tid < qt && i < nq) {
+    const int a = x[i], b = y[i];
+    QMeta m;
+    m.rx = m.ry = m.e0 = 0;
+    m.w = 0;
+    if ((unsigned)a >= (unsigned)q.n || (unsigned)b >= (unsigned)q.n) {
+      m.node = -2;  // id out of range
+    } else if (a == b) {
+      m.node = -1;
+    } else {
+      const Lca L = find_lca(q, a, b);
+      m.node = L.node;
+      m.rx = q.dis_hash[a];
+      m.ry = q.dis_hash[b];
+      if (kind == H2H_QUERY_LCA) {
+        m.e0 = q.pos_hash[L.node];
+        m.w = (int)(q.pos_hash[L.node + 1] - m.e0);
+      } else {
+        m.w = L.depth;
+      }
+    }
+    c = m.w <= 8 ? 0 : (m.w <= 48 ? 1 : 2);
+    meta[tid] = m;
   }
+  __syncthreads();
+  // class lists: wave ballots, one LDS atomic per wave and class
+  const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  for (int k = 0; k < 3; k++) {
+    const uint64_t mk = __ballot(c == k);
+    if (!mk) continue;
+    const int leader = __ffsll((unsigned long long)mk) - 1;
+    int at = 0;
+    if (lane == leader) at = atomicAdd(&cnt[k], __popcll(mk));
+    at = __shfl(at, leader, 64);
+    if (c == k) list[k][at + __popcll(mk & lt)] = (int16_t)tid;
+  }
+  __syncthreads();
+  tile_pass<64>(q, kind, meta, list[2], cnt[2], base, out);
+  tile_pass<16>(q, kind, meta, list[1], cnt[1], base, out);
+  tile_pass<4>(q, kind, meta, list[0], cnt[0], base, out);
 }
 
 }  // namespace
@@ -255,7 +244,7 @@ h2h_status build_lca(h2h_ctx* ctx) {
 }
 
 h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, const int32_t* dy, int32_t* dout,
-                        int64_t nq, hipStream_t s, QueryWorkspace* ws) {
+                        int64_t nq, hipStream_t s) {
   if (nq <= 0) return H2H_OK;
   QArgs q;
   q.tin = ctx->d_tin.as<int32_t>();
@@ -265,37 +254,21 @@ h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, con
   q.pos_pos = ctx->d_pos_pos.as<int32_t>();
   q.dis = ctx->d_dis.as<int32_t>();
   q.n = ctx->n;
-  const unsigned blocks = (unsigned)((nq + 255) / 256);
   if (exec == H2H_EXEC_PLAIN) {
+    const unsigned blocks = (unsigned)((nq + 255) / 256);
     hipLaunchKernelGGL(query_plain_kernel, dim3(blocks), dim3(256), 0, s, q, dx, dy, dout, nq, kind);
     H2H_HIP(ctx, hipGetLastError());
     return H2H_OK;
   }
-  if (!ws || ws->cap < nq) return set_err(ctx, H2H_ERR_ARG, "batched query workspace too small");
-  int32_t* lca_node = ws->lca.as<int32_t>();
-  int32_t* cls = ws->cls.as<int32_t>();
-  int32_t* order = ws->order.as<int32_t>();
-  int32_t* ctr = ws->ctr.as<int32_t>();  // [0,3) counts, [3,6) bases, [6,9) fill
-  hipLaunchKernelGGL(query_zero_kernel, dim3(1), dim3(64), 0, s, ctr, 9);
-  hipLaunchKernelGGL(query_size_kernel, dim3(blocks), dim3(256), 0, s, q, dx, dy, nq, kind, lca_node, cls, ctr);
-  hipLaunchKernelGGL(query_bases_kernel, dim3(1), dim3(64), 0, s, ctr, ctr + 3, ctr + 6);
-  hipLaunchKernelGGL(query_scatter_kernel, dim3(blocks), dim3(256), 0, s, cls, nq, ctr + 3, ctr + 6, order);
-  unsigned grid = (unsigned)std::min<int64_t>((nq + 3) / 4, (int64_t)ctx->num_cus * 8);
-  if (grid == 0) grid = 1;
-  hipLaunchKernelGGL(query_group_kernel, dim3(grid), dim3(256), 0, s, q, dx, dy, dout, lca_node, order, ctr, kind);
+  // queries per workgroup: 256, halved while the batch gives fewer than two
+  // workgroups per CU (down to 16)
+  int qt = 256;
+  while (qt > 16 && (nq + qt - 1) / qt < 2 * (int64_t)ctx->num_cus) qt >>= 1;
+  const unsigned tiles = (unsigned)((nq + qt - 1) / qt);
+  hipLaunchKernelGGL(query_tile_kernel, dim3(tiles), dim3(256), 0, s, q, dx, dy, dout, nq, kind, qt);
   H2H_HIP(ctx, hipGetLastError());
   return H2H_OK;
 }
 
-h2h_status query_workspace(h2h_ctx* ctx, QueryWorkspace& ws, int64_t nq) {
-  if (ws.cap >= nq) return H2H_OK;
-  const int64_t cap = std::max<int64_t>(nq, 1024);
-  H2H_TRY(dev_alloc(ctx, ws.lca, cap * sizeof(int32_t)));
-  H2H_TRY(dev_alloc(ctx, ws.cls, cap * sizeof(int32_t)));
-  H2H_TRY(dev_alloc(ctx, ws.order, cap * sizeof(int32_t)));
-  H2H_TRY(dev_alloc(ctx, ws.ctr, 16 * sizeof(int32_t)));
-  ws.cap = cap;
-  return H2H_OK;
-}
 
 }  // namespace h2h

@@ -34,7 +34,6 @@ struct Lane {
   hipEvent_t done = nullptr;
   int32_t *hx = nullptr, *hy = nullptr, *hout = nullptr;  // pinned
   DevArray dx, dy, dout;
-  QueryWorkspace ws;
   std::vector<std::shared_ptr<Request>> batch;
   std::vector<int64_t> offs;
   int64_t count = 0;
@@ -122,7 +121,7 @@ void Server::run() {
         h2h_status st = H2H_OK;
         if (e == hipSuccess)
           st = launch_query(ctx, cfg.kind, cfg.exec, L.dx.as<int32_t>(), L.dy.as<int32_t>(), L.dout.as<int32_t>(), cnt,
-                            L.stream, &L.ws);
+                            L.stream);
         if (e == hipSuccess && st == H2H_OK)
           e = hipMemcpyAsync(L.hout, L.dout.p, cnt * sizeof(int32_t), hipMemcpyDeviceToHost, L.stream);
         if (e == hipSuccess) e = hipEventRecord(L.done, L.stream);
@@ -212,7 +211,6 @@ h2h_status h2h_server_start(h2h_ctx* ctx, const h2h_server_cfg* cfg) {
     if ((st = dev_alloc(ctx, L.dx, bytes))) return st;
     if ((st = dev_alloc(ctx, L.dy, bytes))) return st;
     if ((st = dev_alloc(ctx, L.dout, bytes))) return st;
-    if ((st = query_workspace(ctx, L.ws, cfg->max_request))) return st;
   }
   S->t0 = std::chrono::steady_clock::now();
   Server* raw = S.get();
