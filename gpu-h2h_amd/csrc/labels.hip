@@ -1889,12 +1889,13 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   LabelPlan* pp = nullptr;
   H2H_TRY(prepare_build(ctx, kind, method, a, pp));
   LabelPlan& P = *pp;
-  if (method == H2H_METHOD_BATCHED && P.graph[l4] && P.graph_dis[l4] != ctx->d_dis.p) {
+  if (P.graph[l4] && P.graph_dis[l4] != ctx->d_dis.p) {
     (void)hipGraphExecDestroy(P.graph[l4]);
     P.graph[l4] = nullptr;
   }
-  if (method == H2H_METHOD_BATCHED && !P.graph[l4]) {
-    // capture the whole build once; replays skip per-launch host work
+  if (!P.graph[l4]) {
+    // capture the whole build once (the level sweeps' one launch per tree
+    // level too); replays skip per-launch host work
     hipGraph_t g = nullptr;
     H2H_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const h2h_status st = enqueue_plan(ctx, P, a, s, nullptr);
@@ -1913,11 +1914,7 @@ h2h_status run_label_build(h2h_ctx* ctx, int kind, int method, double* usec) {
   H2H_HIP(ctx, hipEventCreate(&e0));
   H2H_HIP(ctx, hipEventCreate(&e1));
   H2H_HIP(ctx, hipEventRecord(e0, s));
-  if (method == H2H_METHOD_BATCHED) {
-    H2H_HIP(ctx, hipGraphLaunch(P.graph[l4], s));
-  } else {
-    H2H_TRY(enqueue_plan(ctx, P, a, s, nullptr));
-  }
+  H2H_HIP(ctx, hipGraphLaunch(P.graph[l4], s));
   H2H_HIP(ctx, hipGetLastError());
   H2H_HIP(ctx, hipEventRecord(e1, s));
   H2H_HIP(ctx, hipEventSynchronize(e1));
