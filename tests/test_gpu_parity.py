@@ -175,6 +175,22 @@ def test_batched_plan_shapes(monkeypatch, task_nodes, chain_nodes, levels, chunk
                     np.testing.assert_array_equal(p.labels()["dis"], o.dis())
 
 
+@pytest.mark.parametrize("tri,prune,ra", [(0, 1, 8), (1, 0, 8), (1, 1, 16), (0, 0, 16)])
+def test_batched_alternative_paths(monkeypatch, tri, prune, ra):
+    """The batched build's switchable forms give the same labels: chain
+    triangles by Floyd-Warshall instead of the min-plus product, every bag
+    entry instead of leaving out the two-hop dominated ones, and the task
+    apply at 16 columns per lane."""
+    monkeypatch.setenv("H2H_TRI_PRODUCT", str(tri))
+    monkeypatch.setenv("H2H_PRUNE", str(prune))
+    monkeypatch.setenv("H2H_TASK_RA", str(ra))
+    for case, C in ((cases.grid64(), 1), (cases.delaunay(30000, 2.6, 5, 7), 2)):
+        o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=4)
+        with p:
+            for kind in (h2h.L4, h2h.L3):
+                _compare_labels(o, p, kind, h2h.BATCHED)
+
+
 def test_fla1m_batched_matches_plain_and_dijkstra():
     """The 1.07M-vertex FLA-scale workload: the batched L4 build equals the
     per-node build entry for entry, and sampled rows of both query kinds
