@@ -261,31 +261,22 @@ h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int
   invalidate_label_plans(ctx);
   ctx->change_height = change_height;
   ctx->contract_order = contract_order;
-  // H2H_HOST_TIMING=1 prints the host phases of the contraction to stderr
-  const bool timing = std::getenv("H2H_HOST_TIMING") != nullptr;
-  auto now = [] { return std::chrono::steady_clock::now(); };
-  auto t0 = now();
-  auto lap = [&](const char* what) {
-    if (!timing) return;
-    const auto t1 = now();
-    std::fprintf(stderr, "h2h_contract: %-22s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(t1 - t0).count());
-    t0 = t1;
-  };
+  host_lap("contract: start");
   h2h_status st = build_partition_plans(ctx, change_height);
   if (st) return st;
-  lap("partition plans");
+  host_lap("partition plans");
   st = run_contraction(ctx, usec);
   if (st) return st;
-  lap("contraction");
+  host_lap("contraction");
   ctx->plans.clear();
   ctx->plans.shrink_to_fit();
   if (contract_order == H2H_CONTRACT_CPU_ORDER) merge_cpu_order(ctx, thread_num);
   st = build_tree_and_layout(ctx);
   if (st) return st;
-  lap("tree and layout");
+  host_lap("tree and layout");
   st = upload_index(ctx);
   if (st) return st;
-  lap("index upload, LCA");
+  host_lap("index upload, LCA");
   ctx->contracted = true;
   return H2H_OK;
 }

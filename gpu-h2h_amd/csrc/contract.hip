@@ -412,6 +412,7 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
   H2H_HIP(ctx, hipMemsetAsync(d_blen.p, 0, n * sizeof(int32_t), s));
   H2H_HIP(ctx, hipMemsetAsync(d_err.p, 0, sizeof(int32_t), s));
 
+  host_lap("  contraction: rows uploaded");
   // ---- per-plan inputs ----
   int64_t wtot = 0;
   std::vector<int64_t> wbase(ctx->plans.size());
@@ -481,6 +482,7 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     hipLaunchKernelGGL(contract_kernel, dim3((unsigned)P.heap.size()), dim3(D.block), D.shmem, s, a);
     H2H_HIP(ctx, hipGetLastError());
   }
+  host_lap("  contraction: levels launched");
   // ---- bags: sizes -> offsets -> extraction ----
   std::vector<int32_t> blen(n);
   H2H_HIP(ctx, hipMemcpyAsync(blen.data(), d_blen.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -529,6 +531,7 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
   (void)hipEventDestroy(e1);
   ctx->t_contract_us = ms * 1000.0;
   if (usec) *usec = ctx->t_contract_us;
+  host_lap("  contraction: bags downloaded");
   // dominated entries come back as -(w + 1)
   ctx->bag_dom.assign(nb, 0);
   for (int64_t e = 0; e < nb; e++)

@@ -5,8 +5,11 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <map>
 #include <memory>
@@ -21,6 +24,16 @@
 namespace h2h {
 
 constexpr int32_t INF32 = 0x7fffffff;
+
+// H2H_HOST_TIMING=1: host phases print their time since the previous lap to
+// stderr (diagnostics only)
+inline void host_lap(const char* what) {
+  static const bool on = std::getenv("H2H_HOST_TIMING") != nullptr;
+  static thread_local std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+  const auto now = std::chrono::steady_clock::now();
+  if (on) std::fprintf(stderr, "h2h host: %-28s %8.2f ms\n", what, std::chrono::duration<double, std::milli>(now - last).count());
+  last = now;
+}
 
 // Owning device allocation.
 struct DevArray {

@@ -284,6 +284,7 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
     return set_err(ctx, H2H_ERR_INPUT, "elimination tree is a forest (graph disconnected)");
   ctx->tree_height = maxd;
   ctx->label_entries = sumd;
+  host_lap("  tree: fathers, DFS");
 
   // label layout
   ctx->dis_hash.assign(n + 1, 0);
@@ -297,6 +298,7 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
   const bool dom = ctx->bag_dom.size() == ctx->bag_nbr.size() && !ctx->bag_dom.empty();
   if (dom) ctx->pos_dom.assign(P, 0);
   else ctx->pos_dom.clear();
+  host_lap("  tree: layout arrays");
   // each node's bag sorted by position; nodes are independent, so the host
   // threads take contiguous node ranges
   auto layout = [&](int v0, int v1) {
@@ -328,6 +330,7 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
     layout(0, (int)((int64_t)n / T));
     for (auto& th : pool) th.join();
   }
+  host_lap("  tree: sorted bags");
   // BFS levels (TreeBFS / TreeHash)
   ctx->bfs.clear();
   ctx->bfs.reserve(n);

@@ -69,6 +69,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 #include "h2h_internal.h"
 
@@ -1301,6 +1302,7 @@ h2h_status plan_items(h2h_ctx* ctx, int method, LabelPlan& P) {
 
 h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   const int n = ctx->n;
+  host_lap("plan: start");
   if (ctx->tree_height >= (1 << 25))
     return set_err(ctx, H2H_ERR_INPUT, "batched plan: tree deeper than 2^25 levels");
   // maximal subtrees of at most smax nodes and TASK_H levels become tasks
@@ -1326,6 +1328,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     }
   }
 
+  host_lap("  plan: tasks selected");
   // chains over the other nodes, built in BFS order (fathers first): a node
   // extends its father's chain when it is the father's only child outside
   // the tasks and the chain has room
@@ -1362,6 +1365,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     chain_of[v] = c;
     hc[c].nodes.push_back(v);
   }
+  host_lap("  plan: chains");
   std::vector<int> corder(hc.size());
   for (size_t c = 0; c < hc.size(); c++) corder[c] = (int)c;
   std::stable_sort(corder.begin(), corder.end(), [&](int x, int y) { return hc[x].cd < hc[y].cd; });
@@ -1423,6 +1427,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       maxdepth = std::max(maxdepth, ctx->depth[v]);
     }
   }
+  host_lap("  plan: chain entries");
   // work units of the four chain kernels, chain depth by chain depth
   std::vector<int4> units;
   std::vector<BoundUnit> bunits;
@@ -1492,6 +1497,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   H2H_TRY(upload(ctx, P.tpath, tpath.data(), tpath.size() * sizeof(int32_t)));
   H2H_TRY(upload(ctx, P.cunits, units.data(), units.size() * sizeof(int4)));
 
+  host_lap("  plan: chain units, uploads");
   // tasks: bound units, closure lists and apply units (outer columns), BFS
   // levels (closure and inner columns)
   {
@@ -1508,6 +1514,27 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     const int ra = (int)env_int("H2H_TASK_RA", 8, 8, 16) == 16 ? 16 : 8;
     P.task_ra = ra;
     std::vector<int32_t> local(n, -1);
+    {
+      // reserve the per-node outputs (task depth sums)
+      int64_t nodes = 0, dsum = 0;
+      for (int t = 0; t < ntasks; t++) {
+        const int r = ctx->depth[members[t][0]];
+        nodes += (int64_t)members[t].size();
+        for (int v : members[t]) dsum += ctx->depth[v] - r;
+      }
+      bnodes.reserve(nodes);
+      banc.reserve(nodes);
+      tanc.reserve(dsum);
+      tanl.reserve(dsum);
+      cl_dst.reserve(dsum + nodes);
+      cl_rc.reserve(dsum + nodes);
+      aunits.reserve(nodes * 2);
+      bunits.reserve(bunits.size() + nodes);
+    }
+    std::vector<BoundUnit> tb_units;
+    std::vector<int4> ta_units;
+    std::vector<uint8_t> has_outer;
+    std::vector<int> anc_local, anc_v;
     P.tbound_b = (int)bunits.size();
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
     // while they are still in the Infinity Cache
@@ -1527,11 +1554,11 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       troot.push_back(r);
       tinfo.push_back(make_int4(r, fc.path, fc.first, fc.h0));
       for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
-      std::vector<BoundUnit> tb_units;
-      std::vector<int4> ta_units;
+      tb_units.clear();
+      ta_units.clear();
       // members with entries above the task: only their bounds are finite,
       // so only they get bound units and closure entries
-      std::vector<uint8_t> has_outer(mem.size(), 0);
+      has_outer.assign(mem.size(), 0);
       for (size_t k = 0; k < mem.size(); k++) {
         const int v = mem[k];
         const int d = ctx->depth[v] - r;
@@ -1548,7 +1575,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         const int sel0 = (int)cl_rc.size();
         {
           int x = v;
-          std::vector<int> anc_local(d + 1), anc_v(d + 1);
+          anc_local.resize(d + 1);
+          anc_v.resize(d + 1);
           anc_local[d] = (int)k;
           anc_v[d] = v;
           for (int q = d - 1; q >= 0; q--) {
@@ -1613,6 +1641,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       }
       for (int v : mem) local[v] = -1;
     }
+    host_lap("  plan: task lists");
     P.tbound_n = (int)bunits.size() - P.tbound_b;
     P.tchunk_b.push_back((int)bunits.size());
     P.tchunk_a.push_back((int)aunits.size());
@@ -1640,29 +1669,38 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     P.task_nunits = (int)aunits.size();
   }
   H2H_TRY(upload(ctx, P.bunits, bunits.data(), bunits.size() * sizeof(BoundUnit)));
+  host_lap("  plan: task uploads");
   P.pruned = prune;
   if (prune) {
     const int64_t PE = ctx->pos_hash[n];
     std::vector<int32_t> bp(PE), bw(PE);
     std::vector<int64_t> br(PE);
-    for (int v = 0; v < n; v++) {
-      const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1];
-      int64_t k = e0;
-      for (int pass = 0; pass < 3; pass++)
-        for (int64_t e = e0; e < e1; e++) {
-          const bool outer = e < e1 - 1 && ctx->pos_pos[e] < thr[v];
-          const int cls = outer ? (ctx->pos_dom[e] ? 1 : 0) : 2;
-          if (cls != pass) continue;
-          bp[k] = ctx->pos_pos[e];
-          bw[k] = ctx->pos_w[e];
-          br[k] = ctx->dis_hash[ctx->pos_id[e]];
-          k++;
-        }
-    }
+    auto copy = [&](int v0, int v1) {
+      for (int v = v0; v < v1; v++) {
+        const int64_t e0 = ctx->pos_hash[v], e1 = ctx->pos_hash[v + 1];
+        int64_t k = e0;
+        for (int pass = 0; pass < 3; pass++)
+          for (int64_t e = e0; e < e1; e++) {
+            const bool outer = e < e1 - 1 && ctx->pos_pos[e] < thr[v];
+            const int cls = outer ? (ctx->pos_dom[e] ? 1 : 0) : 2;
+            if (cls != pass) continue;
+            bp[k] = ctx->pos_pos[e];
+            bw[k] = ctx->pos_w[e];
+            br[k] = ctx->dis_hash[ctx->pos_id[e]];
+            k++;
+          }
+      }
+    };
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < T; i++) pool.emplace_back(copy, (int)((int64_t)n * i / T), (int)((int64_t)n * (i + 1) / T));
+    copy(0, (int)((int64_t)n / T));
+    for (auto& th : pool) th.join();
     H2H_TRY(upload(ctx, P.bpos_pos, bp.data(), PE * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.bpos_w, bw.data(), PE * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.bpos_rowoff, br.data(), PE * sizeof(int64_t)));
   }
+  host_lap("  plan: entry copy");
   auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };
   P.nt = nt;
   P.tbo = up64(ctx->label_entries);
@@ -1867,6 +1905,7 @@ h2h_status upload_index(h2h_ctx* ctx) {
   const int64_t P = ctx->pos_hash[n];
   std::vector<int64_t> rowoff(P);
   for (int64_t k = 0; k < P; k++) rowoff[k] = ctx->dis_hash[ctx->pos_id[k]];
+  host_lap("  upload: row offsets");
   H2H_TRY(upload(ctx, ctx->d_dis_hash, ctx->dis_hash.data(), (n + 1) * sizeof(int64_t)));
   H2H_TRY(upload(ctx, ctx->d_pos_hash, ctx->pos_hash.data(), (n + 1) * sizeof(int64_t)));
   H2H_TRY(upload(ctx, ctx->d_pos_pos, ctx->pos_pos.data(), P * sizeof(int32_t)));
