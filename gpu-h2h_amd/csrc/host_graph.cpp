@@ -231,13 +231,22 @@ h2h_status build_tree_and_layout(h2h_ctx* ctx) {
   for (int i = 0; i < n; i++) ctx->rank[ctx->elim[i]] = i;
   ctx->head = ctx->elim[n - 1];
   ctx->father.assign(n, -1);
-  for (int v = 0; v < n; v++) {
-    int32_t f = -1, r = INT_MAX;
-    for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
-      int32_t u = ctx->bag_nbr[e];
-      if (ctx->rank[u] < r) { r = ctx->rank[u]; f = u; }
+  auto fathers = [&](int v0, int v1) {
+    for (int v = v0; v < v1; v++) {
+      int32_t f = -1, r = INT_MAX;
+      for (int64_t e = ctx->bag_off[v]; e < ctx->bag_off[v + 1]; e++) {
+        int32_t u = ctx->bag_nbr[e];
+        if (ctx->rank[u] < r) { r = ctx->rank[u]; f = u; }
+      }
+      ctx->father[v] = f;
     }
-    ctx->father[v] = f;
+  };
+  {
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < T; i++) pool.emplace_back(fathers, (int)((int64_t)n * i / T), (int)((int64_t)n * (i + 1) / T));
+    fathers(0, (int)((int64_t)n / T));
+    for (auto& th : pool) th.join();
   }
   // children CSR in ascending child id
   std::vector<int32_t> ccount(n + 1, 0), child(std::max(n - 1, 1));

@@ -1900,17 +1900,31 @@ void invalidate_label_plans(h2h_ctx* ctx) {
   for (auto& P : ctx->lplan) P.reset();
 }
 
+__global__ void rowoff_kernel(const int64_t* __restrict__ dis_hash, const int32_t* __restrict__ id,
+                              int64_t* __restrict__ rowoff, int64_t P) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < P) rowoff[k] = dis_hash[id[k]];
+}
+
 h2h_status upload_index(h2h_ctx* ctx) {
   const int n = ctx->n;
   const int64_t P = ctx->pos_hash[n];
-  std::vector<int64_t> rowoff(P);
-  for (int64_t k = 0; k < P; k++) rowoff[k] = ctx->dis_hash[ctx->pos_id[k]];
-  host_lap("  upload: row offsets");
   H2H_TRY(upload(ctx, ctx->d_dis_hash, ctx->dis_hash.data(), (n + 1) * sizeof(int64_t)));
   H2H_TRY(upload(ctx, ctx->d_pos_hash, ctx->pos_hash.data(), (n + 1) * sizeof(int64_t)));
   H2H_TRY(upload(ctx, ctx->d_pos_pos, ctx->pos_pos.data(), P * sizeof(int32_t)));
   H2H_TRY(upload(ctx, ctx->d_pos_w, ctx->pos_w.data(), P * sizeof(int32_t)));
-  H2H_TRY(upload(ctx, ctx->d_pos_rowoff, rowoff.data(), P * sizeof(int64_t)));
+  {
+    // each entry's row offset, dis_hash[pos_id], formed on the device
+    DevArray d_id;
+    H2H_TRY(upload(ctx, d_id, ctx->pos_id.data(), std::max<int64_t>(P, 1) * sizeof(int32_t)));
+    H2H_TRY(dev_alloc(ctx, ctx->d_pos_rowoff, std::max<int64_t>(P, 1) * sizeof(int64_t)));
+    if (P > 0)
+      hipLaunchKernelGGL(rowoff_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, ctx->stream,
+                         ctx->d_dis_hash.as<int64_t>(), d_id.as<int32_t>(), ctx->d_pos_rowoff.as<int64_t>(), P);
+    H2H_HIP(ctx, hipGetLastError());
+    H2H_HIP(ctx, hipStreamSynchronize(ctx->stream));  // before d_id is freed
+  }
+  host_lap("  upload: bags, row offsets");
   H2H_TRY(upload(ctx, ctx->d_bfs, ctx->bfs.data(), n * sizeof(int32_t)));
   H2H_TRY(upload(ctx, ctx->d_father, ctx->father.data(), n * sizeof(int32_t)));
   H2H_TRY(upload(ctx, ctx->d_depth, ctx->depth.data(), n * sizeof(int32_t)));

@@ -222,23 +222,32 @@ __global__ void __launch_bounds__(256) query_tile_kernel(QArgs q, const int32_t*
     if (s_ != H2H_OK) return s_; \
   } while (0)
 
+// one doubling level of the preorder sparse table
+__global__ void lca_level_kernel(const int64_t* __restrict__ prev, int64_t* __restrict__ cur, int n, int half) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cur[i] = (i + 2 * half <= n) ? min(prev[i], prev[i + half]) : 0;
+}
+
 h2h_status build_lca(h2h_ctx* ctx) {
   const int n = ctx->n;
   int K = 1;
   while ((1LL << K) <= n) K++;
-  std::vector<int64_t> t((size_t)K * n, 0);
+  // level 0 on the host (preorder), the K - 1 doubling levels on the device;
+  // entries past the last full window of a level are zero and never read
+  std::vector<int64_t> t0(n);
   for (int i = 0; i < n; i++) {
     const int w = ctx->preorder[i];
-    t[i] = ((int64_t)ctx->depth[w] << 32) | (uint32_t)ctx->father[w];
+    t0[i] = ((int64_t)ctx->depth[w] << 32) | (uint32_t)ctx->father[w];
   }
-  for (int k = 1; k < K; k++) {
-    const int half = 1 << (k - 1);
-    int64_t* cur = t.data() + (size_t)k * n;
-    const int64_t* prev = t.data() + (size_t)(k - 1) * n;
-    for (int i = 0; i + (1 << k) <= n; i++) cur[i] = std::min(prev[i], prev[i + half]);
-  }
+  H2H_TRY(dev_alloc(ctx, ctx->d_lca, (size_t)K * n * sizeof(int64_t)));
+  H2H_HIP(ctx, hipMemcpy(ctx->d_lca.p, t0.data(), (size_t)n * sizeof(int64_t), hipMemcpyHostToDevice));
+  int64_t* d = ctx->d_lca.as<int64_t>();
+  for (int k = 1; k < K; k++)
+    hipLaunchKernelGGL(lca_level_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       d + (size_t)(k - 1) * n, d + (size_t)k * n, n, 1 << (k - 1));
+  H2H_HIP(ctx, hipGetLastError());
   ctx->lca_levels = K;
-  H2H_TRY(upload(ctx, ctx->d_lca, t.data(), t.size() * sizeof(int64_t)));
   H2H_TRY(upload(ctx, ctx->d_tin, ctx->tin.data(), n * sizeof(int32_t)));
   return H2H_OK;
 }
