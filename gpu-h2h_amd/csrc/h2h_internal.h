@@ -96,6 +96,7 @@ struct LabelPlan {
   DevArray chains, tnode, touter, tinner, inner_w, inner_k, inner_i, tpath, cunits;
   struct Round {  // one chain depth: unit ranges of the four chain kernels
     int k1b, k1n, k2b, k2n, k3b, k3n, k4b, k4n, lmax, imax;
+    int clmask;  // closure size classes of this depth's chains (1: 128, 2: 64, 4: 32)
   };
   std::vector<Round> rounds;
   int kcl_b[3] = {0, 0, 0}, kcl_n[3] = {0, 0, 0};  // closure launches (every chain) by size class 128, 64, 32
@@ -137,7 +138,7 @@ struct h2h_ctx {
   hipStream_t stream = nullptr;
   // side stream and events of the batched label build (captured into its graph)
   hipStream_t aux_stream = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join1 = nullptr, ev_join2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_cl[3] = {nullptr, nullptr, nullptr}, ev_join2 = nullptr;
   std::string err;
 
   // ---- graph (Graph::Graph) ----

@@ -1470,6 +1470,10 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       rd.imax = std::max(rd.imax, tinner[ch[ci].first + ch[ci].len] - tinner[ch[ci].first]);
     }
     rd.k4n = (int)units.size() - rd.k4b;
+    for (size_t ci = c0; ci < c1; ci++) {
+      const int S = fw_size(ch[ci].len);
+      rd.clmask |= S == 128 ? 1 : S == 64 ? 2 : 4;
+    }
     P.rounds.push_back(rd);
     c0 = c1;
   }
@@ -1797,9 +1801,12 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     if (s2) {
       H2H_HIP(ctx, hipEventRecord(ctx->ev_fork, s));
       H2H_HIP(ctx, hipStreamWaitEvent(s2, ctx->ev_fork, 0));
-      for (int sc = 0; sc < 3; sc++)
+      // one event per size class: a chain depth waits only for the classes
+      // of its own chains
+      for (int sc = 0; sc < 3; sc++) {
         if (P.kcl_n[sc] > 0) launch_chain_closure(c, P.kcl_b[sc], P.kcl_n[sc], 128 >> sc, s2);
-      H2H_HIP(ctx, hipEventRecord(ctx->ev_join1, s2));
+        H2H_HIP(ctx, hipEventRecord(ctx->ev_cl[sc], s2));
+      }
     } else {
       for (int sc = 0; sc < 3; sc++)
         if (P.kcl_n[sc] > 0) H2H_TRY(timed([&] { launch_chain_closure(c, P.kcl_b[sc], P.kcl_n[sc], 128 >> sc, s); }));
@@ -1814,27 +1821,33 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
       return H2H_OK;
     };
     if (P.tclose_after == 0) H2H_TRY(fork_task_close());
-    bool joined1 = s2 == nullptr;
+    int waited = s2 ? 0 : 7;  // size classes whose closures the main stream has waited for
     for (size_t r = 0; r < P.rounds.size(); r++) {
       const auto& rd = P.rounds[r];
       if ((int)r == P.tclose_after && r > 0) H2H_TRY(fork_task_close());
       const int n[4] = {rd.k1n, rd.k2n, a.l4 ? rd.k3n : 0, rd.k4n};
+      bool joined = waited == 7;  // this depth's closures are in
       for (int step = 0; step < 4; step++) {
         if (n[step] == 0) continue;
         // the closures P are read by the outer-column apply and by the L4
-        // triangle product; chain depths before the first apply (the root
-        // chain) take the Floyd-Warshall triangle instead of waiting
-        if (step == 1 && !joined1) {
-          H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join1, 0));
-          joined1 = true;
+        // triangle product; chain depths with no apply (the root chain)
+        // take the Floyd-Warshall triangle instead of waiting
+        if (step == 1 && !joined) {
+          for (int sc = 0; sc < 3; sc++)
+            if ((rd.clmask >> sc & 1) && !(waited >> sc & 1)) {
+              H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_cl[sc], 0));
+              waited |= 1 << sc;
+            }
+          joined = true;
         }
         ChainArgs cs = c;
-        if ((step == 2 || step == 3) && !joined1) cs.tri = 0;
+        if ((step == 2 || step == 3) && !joined) cs.tri = 0;
         H2H_TRY(timed([&] { launch_chain_round(cs, rd, s, step); }));
       }
     }
     if (P.tclose_after >= (int)P.rounds.size() && P.tclose_after > 0) H2H_TRY(fork_task_close());
-    if (!joined1) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join1, 0));
+    for (int sc = 0; sc < 3; sc++)
+      if (!(waited >> sc & 1)) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_cl[sc], 0));
     if (s2) H2H_HIP(ctx, hipStreamWaitEvent(s, ctx->ev_join2, 0));
     if (P.ntasks > 0) {
       if (P.task_levels) {
@@ -1869,7 +1882,7 @@ h2h_status prepare_build(h2h_ctx* ctx, int kind, int method, LabelArgs& a, Label
   if (method == H2H_METHOD_BATCHED) {
     H2H_HIP(ctx, set_chain_lds_limits());
     if (!ctx->aux_stream) H2H_HIP(ctx, hipStreamCreateWithFlags(&ctx->aux_stream, hipStreamNonBlocking));
-    for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_join1, &ctx->ev_join2})
+    for (hipEvent_t* e : {&ctx->ev_fork, &ctx->ev_cl[0], &ctx->ev_cl[1], &ctx->ev_cl[2], &ctx->ev_join2})
       if (!*e) H2H_HIP(ctx, hipEventCreateWithFlags(e, hipEventDisableTiming));
   }
   a.bfs = ctx->d_bfs.as<int32_t>();

@@ -176,7 +176,7 @@ h2h_ctx::h2h_ctx() = default;
 
 h2h_ctx::~h2h_ctx() {
   server.reset();
-  for (hipEvent_t e : {ev_fork, ev_join1, ev_join2})
+  for (hipEvent_t e : {ev_fork, ev_cl[0], ev_cl[1], ev_cl[2], ev_join2})
     if (e) (void)hipEventDestroy(e);
   if (aux_stream) (void)hipStreamDestroy(aux_stream);
   if (stream) (void)hipStreamDestroy(stream);
