@@ -293,6 +293,129 @@ h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* us
   return run_label_build(ctx, label_kind, method, usec);
 }
 
+h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_result* best, h2h_tune_result* tried,
+                            int max_tried, int* ntried) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (need_gpu(ctx)) return H2H_ERR_NOGPU;
+  if (!ctx->tree_ready) return set_err(ctx, H2H_ERR_STATE, "contract (or upload an index) first");
+  if (label_kind != H2H_LABEL_L3 && label_kind != H2H_LABEL_L4) return set_err(ctx, H2H_ERR_ARG, "label kind must be 3 or 4");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before rebuilding labels");
+  (void)hipSetDevice(ctx->device);
+  replays = std::min(std::max(replays, 1), 50);
+  int nt = 0;
+  // one candidate: a fresh batched plan and graph, then the median of the replays
+  auto measure = [&](const BatchedConfig& c, double& us) -> h2h_status {
+    ctx->bcfg = c;
+    ctx->lplan[H2H_METHOD_BATCHED].reset();
+    double t = 0;
+    h2h_status st = run_label_build(ctx, label_kind, H2H_METHOD_BATCHED, &t);
+    if (st) return st;
+    std::vector<double> v;
+    for (int r = 0; r < replays; r++) {
+      if ((st = run_label_build(ctx, label_kind, H2H_METHOD_BATCHED, &t))) return st;
+      v.push_back(t);
+    }
+    std::sort(v.begin(), v.end());
+    us = v[v.size() / 2];
+    if (tried && nt < max_tried) {
+      h2h_tune_result& o = tried[nt];
+      o.task_nodes = c.task_nodes;
+      o.chain_nodes = c.chain_nodes;
+      o.task_ra = c.task_ra;
+      o.task_chunks = c.task_chunks;
+      o.tri_product = c.tri_product;
+      o.chain_waves = c.chain_waves;
+      o.task_waves = c.task_waves;
+      o.build_us = us;
+    }
+    nt++;
+    return H2H_OK;
+  };
+  // coordinate descent from the defaults: each parameter's alternatives with
+  // the others held at the best so far
+  BatchedConfig cur;
+  double cur_us = 0;
+  h2h_status st = measure(cur, cur_us);
+  if (st) return st;
+  auto try_field = [&](int BatchedConfig::*f, std::initializer_list<int> vals) -> h2h_status {
+    for (int x : vals) {
+      if (cur.*f == x) continue;
+      BatchedConfig c = cur;
+      c.*f = x;
+      double us = 0;
+      h2h_status s2 = measure(c, us);
+      if (s2) return s2;
+      if (us < cur_us) {
+        cur = c;
+        cur_us = us;
+      }
+    }
+    return H2H_OK;
+  };
+  if ((st = try_field(&BatchedConfig::task_nodes, {64, 128, 256}))) return st;
+  if ((st = try_field(&BatchedConfig::chain_nodes, {64}))) return st;
+  if ((st = try_field(&BatchedConfig::task_ra, {16}))) return st;
+  if ((st = try_field(&BatchedConfig::task_chunks, {2, 4}))) return st;
+  if ((st = try_field(&BatchedConfig::chain_waves, {1}))) return st;
+  if ((st = try_field(&BatchedConfig::task_waves, {4}))) return st;
+  // keep the winner: plan it again, so that the resident labels come from it
+  ctx->bcfg = cur;
+  ctx->lplan[H2H_METHOD_BATCHED].reset();
+  double t = 0;
+  if ((st = run_label_build(ctx, label_kind, H2H_METHOD_BATCHED, &t))) return st;
+  if (best) {
+    best->task_nodes = cur.task_nodes;
+    best->chain_nodes = cur.chain_nodes;
+    best->task_ra = cur.task_ra;
+    best->task_chunks = cur.task_chunks;
+    best->tri_product = cur.tri_product;
+    best->chain_waves = cur.chain_waves;
+    best->task_waves = cur.task_waves;
+    best->build_us = cur_us;
+  }
+  if (ntried) *ntried = nt;
+  return H2H_OK;
+}
+
+h2h_status h2h_get_batched_config(h2h_ctx* ctx, h2h_tune_result* cfg) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (!cfg) return set_err(ctx, H2H_ERR_ARG, "null config");
+  const BatchedConfig& c = ctx->bcfg;
+  cfg->task_nodes = c.task_nodes;
+  cfg->chain_nodes = c.chain_nodes;
+  cfg->task_ra = c.task_ra;
+  cfg->task_chunks = c.task_chunks;
+  cfg->tri_product = c.tri_product;
+  cfg->chain_waves = c.chain_waves;
+  cfg->task_waves = c.task_waves;
+  cfg->build_us = 0;
+  return H2H_OK;
+}
+
+h2h_status h2h_set_batched_config(h2h_ctx* ctx, const h2h_tune_result* cfg) {
+  if (need_ctx(ctx)) return H2H_ERR_ARG;
+  if (!cfg) return set_err(ctx, H2H_ERR_ARG, "null config");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before changing the batched plan");
+  auto in = [](int x, int lo, int hi) { return x >= lo && x <= hi; };
+  if (!in(cfg->task_nodes, 0, 256) || !in(cfg->chain_nodes, 1, 128) || (cfg->task_ra != 8 && cfg->task_ra != 16) ||
+      !in(cfg->task_chunks, 1, 64) || !in(cfg->tri_product, 0, 1) || (cfg->chain_waves != 1 && cfg->chain_waves != 4) ||
+      (cfg->task_waves != 1 && cfg->task_waves != 4))
+    return set_err(ctx, H2H_ERR_ARG,
+                   "batched config: task_nodes 0..256, chain_nodes 1..128, task_ra 8 or 16, task_chunks 1..64, "
+                   "tri_product 0 or 1, chain_waves and task_waves 1 or 4");
+  BatchedConfig c;
+  c.task_nodes = cfg->task_nodes;
+  c.chain_nodes = cfg->chain_nodes;
+  c.task_ra = cfg->task_ra;
+  c.task_chunks = cfg->task_chunks;
+  c.tri_product = cfg->tri_product;
+  c.chain_waves = cfg->chain_waves;
+  c.task_waves = cfg->task_waves;
+  ctx->bcfg = c;
+  ctx->lplan[H2H_METHOD_BATCHED].reset();
+  return H2H_OK;
+}
+
 h2h_status h2h_build_labels_profiled(h2h_ctx* ctx, int label_kind, int method, double* kernel_us, int64_t* launches) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;

@@ -131,6 +131,17 @@ struct LabelPlan {
 struct Request;   // serving
 struct Server;
 
+// Batched-build parameters: defaults, or what h2h_tune_batched chose
+struct BatchedConfig {
+  int task_nodes = 0;  // 0: min(256, max(64, n / (8 CUs)))
+  int chain_nodes = 128;
+  int task_ra = 8;
+  int task_chunks = 1;
+  int tri_product = 1;
+  int chain_waves = 4;  // wavefronts per chain node's bound unit (1 or 4)
+  int task_waves = 1;   // wavefronts per task node's bound unit (1 or 4)
+};
+
 }  // namespace h2h
 
 struct h2h_ctx {
@@ -187,6 +198,7 @@ struct h2h_ctx {
   h2h::DevArray d_qx, d_qy, d_qout;   // staging for host-buffer queries
   size_t q_cap = 0;
   h2h::LabelPlan lplan[4];            // indexed by H2H_METHOD_*
+  h2h::BatchedConfig bcfg;            // batched-build parameters
   int lca_levels = 0;
   int num_cus = 256;
 

@@ -319,6 +319,7 @@ struct ChainArgs {
   int64_t pro;             // closure triangles P (scratch inside dis)
   int begin, count;        // units of this launch
   int tri;                 // L4 triangles as the min-plus product (else Floyd-Warshall)
+  int bound_split;         // bound units of chain nodes: four wavefronts per node (else one)
 };
 
 // blockIdx -> work index so that each XCD takes a contiguous run (see above)
@@ -880,7 +881,7 @@ void launch_chain_round(const ChainArgs& base, const LabelPlan::Round& rd, hipSt
   ChainArgs c = base;
   const bool l4 = c.a.l4 != 0, i32 = c.a.idx32 != 0;
   if (step == 0 && rd.k1n > 0) {
-    launch_bound(c.a, c.bunits, c.tpath, rd.k1b, rd.k1n, s, true);
+    launch_bound(c.a, c.bunits, c.tpath, rd.k1b, rd.k1n, s, c.bound_split != 0);
   } else if (step == 1 && rd.k2n > 0) {
     c.begin = rd.k2b;
     c.count = rd.k2n;
@@ -1307,10 +1308,12 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     return set_err(ctx, H2H_ERR_INPUT, "batched plan: tree deeper than 2^25 levels");
   // maximal subtrees of at most smax nodes and TASK_H levels become tasks
   std::vector<int32_t> task_of(n, -1);
+  const BatchedConfig& cfg = ctx->bcfg;
   const int64_t smax = env_int("H2H_TASK_NODES",
-                               std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (8LL * ctx->num_cus))), 1,
-                               TASK_NODES);
-  const int chain_max = (int)env_int("H2H_CHAIN_NODES", CHAIN_MAX, 1, CHAIN_MAX);
+                               cfg.task_nodes > 0 ? std::min<int64_t>(cfg.task_nodes, TASK_NODES)
+                                                  : std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (8LL * ctx->num_cus))),
+                               1, TASK_NODES);
+  const int chain_max = (int)env_int("H2H_CHAIN_NODES", std::min(std::max(cfg.chain_nodes, 1), CHAIN_MAX), 1, CHAIN_MAX);
   int ntasks = 0;
   {
     std::vector<int64_t> sub(n, 1);
@@ -1515,7 +1518,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     std::vector<int4> aunits, tinfo;
     std::vector<int32_t> cl_dst;  // per (node, task depth): index in the apply's closure entries, or -1
     std::vector<int64_t> cl_rc;   // the apply's closure entries: rows
-    const int ra = (int)env_int("H2H_TASK_RA", 8, 8, 16) == 16 ? 16 : 8;
+    const int ra = (int)env_int("H2H_TASK_RA", cfg.task_ra, 8, 16) == 16 ? 16 : 8;
     P.task_ra = ra;
     std::vector<int32_t> local(n, -1);
     {
@@ -1542,7 +1545,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     P.tbound_b = (int)bunits.size();
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
     // while they are still in the Infinity Cache
-    const int nchunks = (int)env_int("H2H_TASK_CHUNKS", 1, 1, 64);
+    const int nchunks = (int)env_int("H2H_TASK_CHUNKS", std::min(std::max(cfg.task_chunks, 1), 64), 1, 64);
     P.tchunk_b.clear();
     P.tchunk_a.clear();
     for (int t = 0; t < ntasks; t++) {
@@ -1768,7 +1771,9 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     c.tro = P.tro;
     c.pro = P.pro;
     c.begin = c.count = 0;
-    c.tri = env_int("H2H_TRI_PRODUCT", 1, 0, 1) != 0 ? 1 : 0;
+    c.tri = env_int("H2H_TRI_PRODUCT", ctx->bcfg.tri_product ? 1 : 0, 0, 1) != 0 ? 1 : 0;
+    c.bound_split = ctx->bcfg.chain_waves > 1 ? 1 : 0;
+    const bool task_split = ctx->bcfg.task_waves > 1;
     TaskArgs t;
     t.a = a;
     t.lvl_base = P.task_lvl_base.as<int32_t>();
@@ -1858,7 +1863,7 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
       for (size_t k = 0; k + 1 < P.tchunk_b.size(); k++) {
         const int b0 = P.tchunk_b[k], b1 = P.tchunk_b[k + 1];
         if (b1 > b0)
-          H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), b0, b1 - b0, s, false); }));
+          H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), b0, b1 - b0, s, task_split); }));
         t.ubegin = P.tchunk_a[k];
         t.nunits = P.tchunk_a[k + 1] - P.tchunk_a[k];
         if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));

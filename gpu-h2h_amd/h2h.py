@@ -47,6 +47,15 @@ class ServerCfg(ctypes.Structure):
                 ("kind", ctypes.c_int), ("exec", ctypes.c_int)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("task_nodes", ctypes.c_int), ("chain_nodes", ctypes.c_int), ("task_ra", ctypes.c_int),
+                ("task_chunks", ctypes.c_int), ("tri_product", ctypes.c_int), ("chain_waves", ctypes.c_int),
+                ("task_waves", ctypes.c_int), ("build_us", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class ReqStats(ctypes.Structure):
     _fields_ = [("submit_us", ctypes.c_double), ("start_us", ctypes.c_double), ("done_us", ctypes.c_double),
                 ("n", ctypes.c_int64)]
@@ -76,6 +85,10 @@ def lib():
             "h2h_contract": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp],
             "h2h_build_labels": [P, ctypes.c_int, ctypes.c_int, dp],
             "h2h_build_labels_profiled": [P, ctypes.c_int, ctypes.c_int, dp, i64p],
+            "h2h_tune_batched": [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(TuneResult),
+                                 ctypes.POINTER(TuneResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+            "h2h_get_batched_config": [P, ctypes.POINTER(TuneResult)],
+            "h2h_set_batched_config": [P, ctypes.POINTER(TuneResult)],
             "h2h_labels_upload": [P, ctypes.c_int32, i64p, i32p, i64p, i32p, i32p, i32p, i32p],
             "h2h_query": [P, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, ctypes.c_int64, dp],
             "h2h_query_device": [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int64, P],
@@ -183,6 +196,34 @@ class H2HIndex:
         us = ctypes.c_double()
         self._chk(lib().h2h_build_labels(self.h, int(kind), int(method), ctypes.byref(us)))
         return us.value
+
+    def tune_batched(self, kind=L4, replays=5):
+        """Autotunes the batched build on the resident index (h2h_tune_batched):
+        returns (best configuration, every measured candidate) as dicts."""
+        best = TuneResult()
+        tried = (TuneResult * 32)()
+        nt = ctypes.c_int()
+        self._chk(lib().h2h_tune_batched(self.h, int(kind), int(replays), ctypes.byref(best), tried, 32,
+                                         ctypes.byref(nt)))
+        return best.as_dict(), [tried[i].as_dict() for i in range(min(nt.value, 32))]
+
+    def batched_config(self):
+        """The context's batched-build parameters (h2h_get_batched_config)."""
+        c = TuneResult()
+        self._chk(lib().h2h_get_batched_config(self.h, ctypes.byref(c)))
+        d = c.as_dict()
+        d.pop("build_us")
+        return d
+
+    def set_batched_config(self, **kw):
+        """Changes some of the batched-build parameters (h2h_set_batched_config);
+        the next batched build plans again."""
+        d = self.batched_config()
+        unknown = set(kw) - set(d)
+        if unknown:
+            raise ValueError(f"unknown batched-build parameters: {sorted(unknown)}")
+        d.update(kw)
+        self._chk(lib().h2h_set_batched_config(self.h, ctypes.byref(TuneResult(**d))))
 
     def build_labels_profiled(self, kind=L4, method=BATCHED):
         """(sum of label-kernel device microseconds, number of launches)."""

@@ -120,6 +120,40 @@ h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int
  * resident.  usec: device time (hipEvent) of the build kernels. */
 h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* usec);
 
+/* Batched-build parameters (H2H_METHOD_BATCHED).  0 in a field of
+ * h2h_tune_result means "automatic" where noted. */
+typedef struct {
+  int task_nodes;   /* subtree task size cap, 1..256 (0: min(256, max(64, n / (8 CUs)))) */
+  int chain_nodes;  /* chain length cap, 1..128 */
+  int task_ra;      /* task apply columns per lane: 8 or 16 */
+  int task_chunks;  /* task bound/apply passes in this many chunks, 1..64 */
+  int tri_product;  /* 1: L4 chain triangles as the min-plus product; 0: Floyd-Warshall */
+  int chain_waves;  /* wavefronts sharing one chain node's bound unit: 1 or 4 */
+  int task_waves;   /* wavefronts sharing one task node's bound unit: 1 or 4 */
+  double build_us;  /* the measured build time of this configuration (h2h_tune_batched) */
+} h2h_tune_result;
+
+/* The project's autotuner for the batched build: times the batched build of
+ * the resident index (median of `replays` graph replays, label kind
+ * `label_kind`) for each candidate configuration -- task size caps, chain
+ * caps, task apply tile widths, task chunking, wavefronts per node in the
+ * bound pass -- keeps the
+ * fastest for every later H2H_METHOD_BATCHED build of this context and
+ * returns it in *best (and up to max_tried measured candidates in tried, if
+ * not NULL; *ntried gets their count).  Every candidate gives identical
+ * labels.  No counterpart in the reference.  Environment variables
+ * (H2H_TASK_NODES ...) still override the chosen values. */
+h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_result* best, h2h_tune_result* tried,
+                            int max_tried, int* ntried);
+
+/* Reads / sets the context's batched-build parameters (what h2h_tune_batched
+ * chose, or one saved from an earlier tuning; build_us is ignored).  Setting
+ * them drops the batched plan, so the next H2H_METHOD_BATCHED build plans
+ * again; values out of range are an H2H_ERR_ARG.  No counterpart in the
+ * reference. */
+h2h_status h2h_get_batched_config(h2h_ctx* ctx, h2h_tune_result* cfg);
+h2h_status h2h_set_batched_config(h2h_ctx* ctx, const h2h_tune_result* cfg);
+
 /* The same build with an event pair around every level launch (no hipGraph):
  * kernel_us is the sum of the label kernels' device durations and launches
  * their count.  No counterpart in the reference: it exists to measure the

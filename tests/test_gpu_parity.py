@@ -191,6 +191,38 @@ def test_batched_alternative_paths(monkeypatch, tri, prune, ra):
                 _compare_labels(o, p, kind, h2h.BATCHED)
 
 
+@pytest.mark.parametrize("cfg", [dict(chain_waves=1, task_waves=4), dict(task_nodes=64, chain_nodes=64, task_chunks=3),
+                                 dict(task_nodes=200, chain_nodes=37, task_ra=16, tri_product=0)])
+def test_batched_config_variants(cfg):
+    """Batched-build parameters set through the C ABI (one wavefront per chain
+    node's bound unit and four per task node's; other task and chain caps;
+    uneven chunks) give the oracle's labels, for both label kinds."""
+    for case, C in ((cases.grid64(), 1), (cases.delaunay(30000, 2.6, 5, 7), 2)):
+        o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=4)
+        with p:
+            p.set_batched_config(**cfg)
+            got = p.batched_config()
+            assert all(got[k] == v for k, v in cfg.items())
+            for kind in (h2h.L4, h2h.L3):
+                _compare_labels(o, p, kind, h2h.BATCHED)
+
+
+def test_tune_batched_keeps_labels():
+    """The autotuner times its candidates on the resident index, keeps the
+    fastest, and every later batched build (the tuned one included) gives
+    the oracle's labels."""
+    for case, C in ((cases.grid64(), 1), (cases.delaunay(30000, 2.6, 5, 7), 2)):
+        o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=4)
+        with p:
+            best, tried = p.tune_batched(h2h.L4, replays=2)
+            assert len(tried) >= 5 and best["build_us"] == min(t["build_us"] for t in tried)
+            o.labels(O.SERIAL, l4=True)
+            np.testing.assert_array_equal(p.labels()["dis"], o.dis())
+            p.build_labels(h2h.L4, h2h.BATCHED)
+            np.testing.assert_array_equal(p.labels()["dis"], o.dis())
+            _compare_labels(o, p, h2h.L3, h2h.BATCHED)
+
+
 def test_fla1m_batched_matches_plain_and_dijkstra():
     """The 1.07M-vertex FLA-scale workload: the batched L4 build equals the
     per-node build entry for entry, and sampled rows of both query kinds

@@ -110,10 +110,29 @@ def test_device_calls_on_host_context(tmp_path):
     with _host() as p:
         p.set_graph(3, u, v, w, np.arange(3), _cut1(3), 1)
         for call in (lambda: p.contract(1), lambda: p.build_labels(), lambda: p.query([0], [1]),
-                     lambda: p.server_start()):
+                     lambda: p.server_start(), lambda: p.tune_batched()):
             with pytest.raises(h2h.H2HError) as e:
                 call()
             assert e.value.status == -8
+
+
+def test_batched_config_round_trip():
+    """The batched-build parameters: defaults, a change read back, and out of
+    range values refused with H2H_ERR_ARG, all without a device."""
+    with _host() as p:
+        d = p.batched_config()
+        assert d == dict(task_nodes=0, chain_nodes=128, task_ra=8, task_chunks=1, tri_product=1, chain_waves=4,
+                         task_waves=1)
+        p.set_batched_config(task_nodes=96, chain_waves=1, task_waves=4)
+        assert p.batched_config() == dict(d, task_nodes=96, chain_waves=1, task_waves=4)
+        for bad in (dict(task_nodes=257), dict(chain_nodes=0), dict(task_ra=12), dict(task_chunks=0),
+                    dict(tri_product=2), dict(chain_waves=2), dict(task_waves=0)):
+            with pytest.raises(h2h.H2HError) as e:
+                p.set_batched_config(**bad)
+            assert e.value.status == -1
+        assert p.batched_config()["task_nodes"] == 96   # a refused change leaves the parameters alone
+        with pytest.raises(ValueError):
+            p.set_batched_config(waves=2)
 
 
 def test_file_errors(tmp_path):
