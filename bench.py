@@ -198,15 +198,23 @@ def serving_sweep(p, n, peak_qps, requests=400, seed=11):
 
 
 def scale_run(cores, steps):
-    """The FLA-scale configuration (1.07M vertices, H=12): batched L4 build
-    against the CPU oracle's, query rates, and the serving sweep."""
+    """The FLA-scale configuration (1.07M vertices, H=12, bottom partitions of
+    uneven size): autotuned batched L4 build against the CPU oracle's, query
+    rates, and the serving sweep."""
     g, ids, cut, H = G.make_workload("fla1m")
     u, v, w = g.arcs()
-    res = {"workload": f"fla1m: {g.n} vertices, {g.m} edges, coordinate bisection H={H}, ChangeToGlobalHeight=1"}
+    cells = cut[(1 << (H - 1)) - 1:, 1] - cut[(1 << (H - 1)) - 1:, 0]
+    res = {"workload": f"fla1m: {g.n} vertices, {g.m} edges, coordinate bisection H={H} with {len(cells)} bottom "
+                       f"partitions of {cells.min()}..{cells.max()} vertices (median {int(np.median(cells))}), "
+                       "ChangeToGlobalHeight=1"}
     with h2h.H2HIndex(0) as p:
         p.set_graph(g.n, u, v, w, ids, cut, H)
         res["contraction_device_s"] = p.contract(1, h2h.CONTRACT_GPU_ORDER, cores) * 1e-6
         entries = p.info()["label_entries"]
+        tuned, tried = p.tune_batched(h2h.L4, replays=3)
+        res["tuned"] = {k: tuned[k] for k in ("task_nodes", "chain_nodes", "task_ra", "task_chunks", "tri_product",
+                                               "chain_waves", "task_waves")}
+        res["tune_candidates"] = len(tried)
         p.build_labels(h2h.L4, h2h.BATCHED)
         t0 = time.perf_counter()
         for _ in range(steps):

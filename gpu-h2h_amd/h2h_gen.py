@@ -133,15 +133,25 @@ def delaunay_graph(n, avg_degree=2.8, seed=1, scale=1e4):
     return Graph(n, e[:, 0], e[:, 1], w, xy)
 
 
-def bisect(xy, height):
+def bisect(xy, height, leaf_range=None, seed=0):
     """Recursive coordinate bisection.  Returns (index_ids, cut) where index_ids
     is the ID array (index -> vertex id) and cut is a (2^H - 1, 2) int32 heap of
-    [l, r) ranges, heap node i at level floor(log2(i+1)) + 1."""
+    [l, r) ranges, heap node i at level floor(log2(i+1)) + 1.
+
+    Cells split at the median, or, with leaf_range=(lo, hi), at a seeded
+    random fraction in [0.3, 0.7] of the sorted cell, narrowed so that every
+    bottom cell ends with lo..hi vertices (uneven partitions)."""
     n = len(xy)
     order = np.arange(n)
     full = (1 << height) - 1
     cut = np.zeros((full, 2), np.int32)
     cut[0] = (0, n)
+    rng = np.random.default_rng(seed)
+    if leaf_range is not None:
+        lo, hi = leaf_range
+        leaves = 1 << (height - 1)
+        if not lo * leaves <= n <= hi * leaves:
+            raise ValueError(f"{n} vertices cannot fill {leaves} cells of {lo}..{hi}")
     for i in range(full):
         level = int(np.floor(np.log2(i + 1)))
         l, r = int(cut[i, 0]), int(cut[i, 1])
@@ -153,6 +163,13 @@ def bisect(xy, height):
         c1 = 2 * i + 1
         if c1 < full:
             mid = (l + r) // 2
+            if leaf_range is not None:
+                s, m = r - l, 1 << (height - 2 - level)   # cell size, bottom cells under each child
+                a, b = max(lo * m, s - hi * m), min(hi * m, s - lo * m)
+                a2, b2 = max(a, int(np.ceil(0.3 * s))), min(b, int(0.7 * s))
+                if a2 <= b2:
+                    a, b = a2, b2
+                mid = l + int(rng.integers(a, b + 1))
             cut[c1] = (l, mid)
             cut[c1 + 1] = (mid, r)
     return order.astype(np.int32), cut
@@ -188,7 +205,7 @@ def write_files(g, index_ids, cut, height, root, name, method="Latitude"):
 WORKLOADS = {
     "grid64": dict(kind="grid", rows=64, cols=64, drop=0.10, seed=1, height=4),
     "ny264k": dict(kind="delaunay", n=264346, avg_degree=2.8, seed=2, height=10),
-    "fla1m": dict(kind="delaunay", n=1070376, avg_degree=2.5, seed=3, height=12),
+    "fla1m": dict(kind="delaunay", n=1070376, avg_degree=2.5, seed=3, height=12, leaf_range=(64, 2048)),
     "e3m6": dict(kind="delaunay", n=3598623, avg_degree=2.4, seed=4, height=14),
 }
 
@@ -197,9 +214,10 @@ def make_workload(name):
     spec = dict(WORKLOADS[name])
     kind = spec.pop("kind")
     height = spec.pop("height")
+    leaf_range = spec.pop("leaf_range", None)
     if kind == "grid":
         g = grid_graph(**spec)
     else:
         g = delaunay_graph(**spec)
-    ids, cut = bisect(g.xy, height)
+    ids, cut = bisect(g.xy, height, leaf_range, seed=spec.get("seed", 0))
     return g, ids, cut, height

@@ -60,3 +60,10 @@ def small_random(seed):
 
 def delaunay(n, deg, seed, H):
     return from_graph(f"del{n}", G.delaunay_graph(n, deg, seed=seed), H)
+
+
+def uneven(n, deg, seed, H, leaf_range):
+    """A Delaunay graph whose bottom partitions have uneven sizes in leaf_range."""
+    g = G.delaunay_graph(n, deg, seed=seed)
+    ids, cut = G.bisect(g.xy, H, leaf_range, seed=seed)
+    return from_graph(f"uneven{n}", g, H, ids, cut)

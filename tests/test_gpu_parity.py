@@ -138,6 +138,26 @@ def test_delaunay_30k(C):
         _compare_queries(o, p, x, y)
 
 
+@pytest.mark.parametrize("order", [h2h.CONTRACT_CPU_ORDER, h2h.CONTRACT_GPU_ORDER])
+def test_uneven_partitions(order):
+    """Bottom partitions of 300..2048 vertices (the largest cells take the
+    contraction's LDS state at its limit): the same order, tree and bags as
+    the oracle, every label build, and both query kinds."""
+    case = cases.uneven(40000, 2.5, 11, 6, (300, 2048))
+    s = case.cut[31:, 1] - case.cut[31:, 0]
+    print(f"uneven cells: {s.min()}..{s.max()}", flush=True)
+    o, p = _contract_both(case, 1, order, threads=8)
+    with p:
+        _compare_structure(o, p, exact_record=True)
+        for kind in (h2h.L4, h2h.L3):
+            for m in METHODS:
+                _compare_labels(o, p, kind, m)
+        rng = np.random.default_rng(13)
+        x = rng.integers(0, case.n, 200000).astype(np.int32)
+        y = rng.integers(0, case.n, 200000).astype(np.int32)
+        _compare_queries(o, p, x, y)
+
+
 def test_labels_upload_roundtrip():
     """Bring-your-own index: the rows mallocH2HLabel_noHub leaves (bag weights,
     0 on the diagonal) plus pos/father/depth, built on the GPU."""

@@ -42,6 +42,31 @@ def test_partition_rank_tree_random(seed):
     np.testing.assert_allclose(frac, ofrac)
 
 
+@pytest.mark.parametrize("H,lo,hi", [(5, 64, 2048), (7, 20, 400), (3, 800, 1400)])
+def test_uneven_bisection(H, lo, hi):
+    """The uneven generator's cut heap: every cell is the union of its two
+    children, bottom cells hold lo..hi vertices, sizes do vary, and the
+    partition rank tree equals the oracle's."""
+    case = cases.uneven(4000, 2.6, 3, H, (lo, hi))
+    cut = case.cut
+    leaves = 1 << (H - 1)
+    for i in range(leaves - 1):
+        assert cut[2 * i + 1, 0] == cut[i, 0] and cut[2 * i + 1, 1] == cut[2 * i + 2, 0]
+        assert cut[2 * i + 2, 1] == cut[i, 1]
+    s = cut[leaves - 1:, 1] - cut[leaves - 1:, 0]
+    assert s.min() >= lo and s.max() <= hi and s.sum() == 4000
+    assert s.max() > s.min()
+    assert sorted(case.ids.tolist()) == list(range(4000))
+    onan, oelim, _ = case.oracle().partition()
+    with _host() as p:
+        case.product(p)
+        nan, elim, _ = p.partition_rank_tree()
+    np.testing.assert_array_equal(nan, onan)
+    np.testing.assert_array_equal(elim, oelim)
+    with pytest.raises(ValueError):
+        G.bisect(np.random.default_rng(0).random((4000, 2)), H, (lo, lo))
+
+
 def _tri():
     # path 0-1-2 plus 2-0, weights 1
     u = np.array([0, 1, 1, 2, 2, 0], np.int32)
