@@ -38,6 +38,7 @@ constexpr int STATE_WORDS = 7;       // per-member state words
 constexpr int DOM_MIN_BAG = 32;      // two-hop marks only for bags this large
 
 struct ContractArgs {
+  const int32_t* porder;     // [parts] partition of each workgroup: most interior vertices first
   const int32_t* memb_off;   // [parts+1]
   const int32_t* members;    // ascending id within a partition
   const int32_t* outcnt;
@@ -75,41 +76,51 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 }
 
 // Fill-in through the eliminated vertex: every neighbour pair (u, x) gets
-// w(u, x) = min(w(u, x), w(v, u) + w(v, x)).  The k x k pair square is
-// spread over the whole workgroup, four pairs per thread per batch with
-// their loads in flight together; each unordered pair (p < q) is visited
-// once, so the new-edge test and the degree increments see no other writer.
+// w(u, x) = min(w(u, x), w(v, u) + w(v, x)).  Only the unordered pairs
+// p < q of the k neighbours are visited, each once, so the new-edge test and
+// the degree increments see no other writer.  The strict upper triangle is
+// folded into k2/2 runs of k2 - 1 pairs (k2 = k rounded up to even; run r
+// holds row r's pairs, then row k2-1-r's), so consecutive threads read
+// consecutive columns of one row.  The block is symmetric (the graph is
+// undirected and every update writes both halves), so one load per pair
+// reads w(u, x) = w(x, u); B pairs per thread have their loads in flight
+// together.
 // MARK: the same pass flags the members whose entry in v's bag a two-hop
 // path through another bag member beats strictly (w(v,x) + w(x,u) <
 // w(v,u)).  Any value w(x,u) takes is the length of a path, so such an entry
 // never gives the minimum of an exact-distance label recurrence.
-template <bool MARK>
-__device__ inline void fill_in(int k, int np, int tid, int T, int32_t* Wp, const int32_t* s_nb, const int32_t* s_nbw,
-                               int32_t* s_deg, int32_t* s_dom) {
-  const int64_t kk = (int64_t)k * k;
-  for (int64_t i0 = tid; i0 < kk; i0 += 4 * T) {
-    int pu[4], px[4], wu[4], wx[4];
-    int32_t cur[4], cxu[4];
-    bool ok[4];
+template <bool MARK, class I>
+__device__ inline void fill_in_pairs(int k, int np, int tid, int T, int32_t* Wp, const int32_t* s_nb,
+                                     const int32_t* s_nbw, int32_t* s_deg, int32_t* s_dom) {
+  constexpr int B = 8;
+  const I k2 = (I)((k + 1) & ~1);
+  const I run = k2 - 1;
+  const I npairs = (k2 / 2) * run;
+  for (I i0 = tid; i0 < npairs; i0 += (I)B * T) {
+    int pu[B], px[B], wu[B], wx[B];
+    int32_t cur[B];
+    bool ok[B];
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int64_t idx = i0 + (int64_t)r * T;
-      const int pp = (int)(idx / max(k, 1)), qq = (int)(idx - (int64_t)pp * k);
-      ok[r] = idx < kk && qq > pp;
-      const int pc = ok[r] ? pp : 0, qc = ok[r] ? qq : 0;
+    for (int r = 0; r < B; r++) {
+      const I idx = i0 + (I)r * T;
+      const I rr = idx / run, c = idx - rr * run;
+      const bool first = c < run - rr;
+      const int p = (int)(first ? rr : k2 - 1 - rr);
+      const int q = (int)(first ? rr + 1 + c : c + 1);
+      ok[r] = idx < npairs && q < k;  // q == k: the padding member of an odd k
+      const int pc = ok[r] ? p : 0, qc = ok[r] ? q : 0;
       pu[r] = s_nb[pc];
       px[r] = s_nb[qc];
       wu[r] = s_nbw[pc];
       wx[r] = s_nbw[qc];
       cur[r] = ok[r] ? Wp[(int64_t)pu[r] * np + px[r]] : 0;
-      cxu[r] = ok[r] ? Wp[(int64_t)px[r] * np + pu[r]] : 0;
     }
 #pragma unroll
-    for (int r = 0; r < 4; r++) {
+    for (int r = 0; r < B; r++) {
       if (!ok[r]) continue;
       const int u = pu[r], x = px[r], ws = wu[r] + wx[r];
       if (MARK) {  // non-negative int32 weights: the uint32 sums cannot wrap
-        if ((uint32_t)wx[r] + (uint32_t)cxu[r] < (uint32_t)wu[r]) s_dom[u] = 1;
+        if ((uint32_t)wx[r] + (uint32_t)cur[r] < (uint32_t)wu[r]) s_dom[u] = 1;
         if ((uint32_t)wu[r] + (uint32_t)cur[r] < (uint32_t)wx[r]) s_dom[x] = 1;
       }
       if (cur[r] == INF32) {
@@ -117,12 +128,21 @@ __device__ inline void fill_in(int k, int np, int tid, int T, int32_t* Wp, const
         Wp[(int64_t)x * np + u] = ws;
         atomicAdd(&s_deg[u], 1);
         atomicAdd(&s_deg[x], 1);
-      } else {
-        if (ws < cur[r]) Wp[(int64_t)u * np + x] = ws;
-        if (ws < cxu[r]) Wp[(int64_t)x * np + u] = ws;
+      } else if (ws < cur[r]) {
+        Wp[(int64_t)u * np + x] = ws;
+        Wp[(int64_t)x * np + u] = ws;
       }
     }
   }
+}
+
+template <bool MARK>
+__device__ inline void fill_in(int k, int np, int tid, int T, int32_t* Wp, const int32_t* s_nb, const int32_t* s_nbw,
+                               int32_t* s_deg, int32_t* s_dom) {
+  if (k < 46341)  // (k2/2)(k2-1) < 2^31
+    fill_in_pairs<MARK, int32_t>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
+  else
+    fill_in_pairs<MARK, int64_t>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
 }
 
 __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
@@ -130,7 +150,7 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
   __shared__ uint64_t red[16];
   __shared__ int32_t s_cnt;
 
-  const int part = blockIdx.x;
+  const int part = a.porder[blockIdx.x];
   const int base = a.memb_off[part];
   const int np = a.memb_off[part + 1] - base;
   const int tid = threadIdx.x;
@@ -304,6 +324,7 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
 // Bags of the eliminated members: the row of v is frozen once v is gone, so
 // its bag is every entry whose member was still present when v was picked.
 struct BagArgs {
+  const int32_t* porder;
   const int32_t* memb_off;
   const int32_t* members;
   const int32_t* outcnt;
@@ -316,7 +337,7 @@ struct BagArgs {
 };
 
 __global__ void __launch_bounds__(1024) extract_bags_kernel(BagArgs a) {
-  const int part = blockIdx.x;
+  const int part = a.porder[blockIdx.x];
   const int base = a.memb_off[part];
   const int np = a.memb_off[part + 1] - base;
   const int lane = threadIdx.x & 63;
@@ -359,7 +380,7 @@ h2h_status up(h2h_ctx* ctx, DevArray& d, const std::vector<T>& h) {
 }
 
 struct PlanDev {
-  DevArray memb_off, members, outcnt, slot, w_off, elim_base, ncand, gstate;
+  DevArray porder, memb_off, members, outcnt, slot, w_off, elim_base, ncand, gstate;
   bool lds = true;
   int block = 256;
   size_t shmem = 0;
@@ -431,6 +452,12 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     for (size_t i = 0; i < P.members.size(); i++) slot[P.members[i]] = (int32_t)i;
     std::vector<int64_t> woff(P.w_off.size());
     for (size_t k = 0; k < woff.size(); k++) woff[k] = P.w_off[k] + wbase[pi];
+    // longest first: the partitions with the most eliminations take the
+    // first workgroup slots
+    std::vector<int32_t> order(P.ncand.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return P.ncand[x] > P.ncand[y]; });
+    H2H_TRY(up(ctx, D.porder, order));
     H2H_TRY(up(ctx, D.memb_off, P.memb_off));
     H2H_TRY(up(ctx, D.members, P.members));
     H2H_TRY(up(ctx, D.outcnt, P.outcnt));
@@ -454,6 +481,7 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     const LevelPlan& P = ctx->plans[pi];
     PlanDev& D = pd[pi];
     ContractArgs a;
+    a.porder = D.porder.as<int32_t>();
     a.memb_off = D.memb_off.as<int32_t>();
     a.members = D.members.as<int32_t>();
     a.outcnt = D.outcnt.as<int32_t>();
@@ -500,6 +528,7 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
   H2H_TRY(dev_alloc(ctx, d_bw, std::max<int64_t>(nb, 1) * sizeof(int32_t)));
   for (size_t pi = 0; pi < ctx->plans.size(); pi++) {
     BagArgs b;
+    b.porder = pd[pi].porder.as<int32_t>();
     b.memb_off = pd[pi].memb_off.as<int32_t>();
     b.members = pd[pi].members.as<int32_t>();
     b.outcnt = pd[pi].outcnt.as<int32_t>();
