@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <numeric>
 #include <tuple>
+#include <type_traits>
 
 #include "h2h_internal.h"
 
@@ -60,7 +61,6 @@ struct ContractArgs {
   int32_t* elim_pos;         // [n] slot of v in elim
   int32_t* bag_len;          // [n]
   int32_t* gstate;           // per-member state when it does not fit in LDS
-  int state_in_lds;
   int stale_first;           // GPU-order rule: first pick on the stale keys
   int track_holes;           // GPU-order rule: slot-list degree
   int32_t* err;              // [1] bit 0: slot capacity, bit 1: key overflow
@@ -84,15 +84,18 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
 // consecutive columns of one row.  The block is symmetric (the graph is
 // undirected and every update writes both halves), so one load per pair
 // reads w(u, x) = w(x, u); B pairs per thread have their loads in flight
-// together.
+// together.  OFF32: the block has fewer than 2^30 words (np < 32768), so
+// offsets into it and pair indices ((k2/2)(k2-1) < 2^29) are 32-bit.
 // MARK: the same pass flags the members whose entry in v's bag a two-hop
 // path through another bag member beats strictly (w(v,x) + w(x,u) <
 // w(v,u)).  Any value w(x,u) takes is the length of a path, so such an entry
 // never gives the minimum of an exact-distance label recurrence.
-template <bool MARK, class I>
+template <bool MARK, bool OFF32>
 __device__ inline void fill_in_pairs(int k, int np, int tid, int T, int32_t* Wp, const int32_t* s_nb,
                                      const int32_t* s_nbw, int32_t* s_deg, int32_t* s_dom) {
-  constexpr int B = 8;
+  using Off = typename std::conditional<OFF32, uint32_t, int64_t>::type;
+  using I = typename std::conditional<OFF32, int32_t, int64_t>::type;
+  constexpr int B = 4;
   const I k2 = (I)((k + 1) & ~1);
   const I run = k2 - 1;
   const I npairs = (k2 / 2) * run;
@@ -113,7 +116,7 @@ __device__ inline void fill_in_pairs(int k, int np, int tid, int T, int32_t* Wp,
       px[r] = s_nb[qc];
       wu[r] = s_nbw[pc];
       wx[r] = s_nbw[qc];
-      cur[r] = ok[r] ? Wp[(int64_t)pu[r] * np + px[r]] : 0;
+      cur[r] = ok[r] ? Wp[(Off)pu[r] * (Off)np + (Off)px[r]] : 0;
     }
 #pragma unroll
     for (int r = 0; r < B; r++) {
@@ -124,13 +127,13 @@ __device__ inline void fill_in_pairs(int k, int np, int tid, int T, int32_t* Wp,
         if ((uint32_t)wu[r] + (uint32_t)cur[r] < (uint32_t)wx[r]) s_dom[x] = 1;
       }
       if (cur[r] == INF32) {
-        Wp[(int64_t)u * np + x] = ws;
-        Wp[(int64_t)x * np + u] = ws;
+        Wp[(Off)u * (Off)np + (Off)x] = ws;
+        Wp[(Off)x * (Off)np + (Off)u] = ws;
         atomicAdd(&s_deg[u], 1);
         atomicAdd(&s_deg[x], 1);
       } else if (ws < cur[r]) {
-        Wp[(int64_t)u * np + x] = ws;
-        Wp[(int64_t)x * np + u] = ws;
+        Wp[(Off)u * (Off)np + (Off)x] = ws;
+        Wp[(Off)x * (Off)np + (Off)u] = ws;
       }
     }
   }
@@ -139,12 +142,16 @@ __device__ inline void fill_in_pairs(int k, int np, int tid, int T, int32_t* Wp,
 template <bool MARK>
 __device__ inline void fill_in(int k, int np, int tid, int T, int32_t* Wp, const int32_t* s_nb, const int32_t* s_nbw,
                                int32_t* s_deg, int32_t* s_dom) {
-  if (k < 46341)  // (k2/2)(k2-1) < 2^31
-    fill_in_pairs<MARK, int32_t>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
+  if ((int64_t)np * np < (1LL << 30))
+    fill_in_pairs<MARK, true>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
   else
-    fill_in_pairs<MARK, int64_t>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
+    fill_in_pairs<MARK, false>(k, np, tid, T, Wp, s_nb, s_nbw, s_deg, s_dom);
 }
 
+// LDS_STATE: the per-member state lives in LDS (a compile-time choice, so
+// that its accesses are LDS instructions rather than generic ones whose
+// waits would also wait for every global load in flight)
+template <bool LDS_STATE>
 __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
   extern __shared__ int32_t smem[];
   __shared__ uint64_t red[16];
@@ -156,11 +163,11 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
   const int tid = threadIdx.x;
   const int T = blockDim.x;
   const int lane = tid & 63;
-  const int wid = tid >> 6;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int nwaves = T >> 6;
   int32_t* Wp = a.W + a.w_off[part];
 
-  int32_t* st = a.state_in_lds ? smem : (a.gstate + (int64_t)STATE_WORDS * base);
+  int32_t* st = LDS_STATE ? smem : (a.gstate + (int64_t)STATE_WORDS * base);
   int32_t* s_deg = st;
   int32_t* s_hgt = st + np;
   int32_t* s_flag = st + 2 * np;
@@ -195,6 +202,7 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
     // ---- pick the minimum key among live candidates ----
     const bool stale = a.stale_first && step == 0;
     uint64_t best = ~0ull;
+    bool ovf = false;
     for (int i = tid; i < np; i += T) {
       if (s_flag[i] == (F_ALIVE | F_CAND)) {
         uint32_t dk, hk;
@@ -207,10 +215,11 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
         }
         const uint32_t dh = dk + hk;
         uint64_t key = ((uint64_t)dh << 43) | ((uint64_t)(dk & 0x1fffff) << 22) | (uint64_t)i;
-        if (dh >= (1u << 21) || dk >= (1u << 21)) atomicOr(a.err, 2);
+        ovf |= dh >= (1u << 21) || dk >= (1u << 21);
         best = key < best ? key : best;
       }
     }
+    if (ovf) atomicOr(a.err, 2);
     best = wave_min_u64(best);
     if (lane == 0) red[wid] = best;
     if (tid == 0) s_cnt = 0;
@@ -221,15 +230,24 @@ __global__ void __launch_bounds__(1024) contract_kernel(ContractArgs a) {
     const int dk = (int)((best >> 22) & 0x1fffff);
     const int hk = (int)(best >> 43) - dk;
     const int32_t gv = a.members[base + v];
-    // ---- gather the neighbours of v (its bag) ----
+    // ---- gather the neighbours of v (its bag): GU row words per thread
+    // loaded together, then appended in any order (nothing downstream
+    // depends on the order of s_nb) ----
     const int32_t* Wv = Wp + (int64_t)v * np;
-    for (int i = tid; i < np; i += T) {
-      if ((s_flag[i] & F_ALIVE) && i != v) {
-        const int32_t w = Wv[i];
-        if (w != INF32) {
-          const int k = atomicAdd(&s_cnt, 1);
-          s_nb[k] = i;
-          s_nbw[k] = w;
+    constexpr int GU = 8;
+    for (int i0 = tid; i0 < np; i0 += GU * T) {
+      int32_t wv[GU];
+#pragma unroll
+      for (int u = 0; u < GU; u++) {
+        const int i = i0 + u * T;
+        wv[u] = (i < np && i != v && (s_flag[i] & F_ALIVE)) ? Wv[i] : INF32;
+      }
+#pragma unroll
+      for (int u = 0; u < GU; u++) {
+        if (wv[u] != INF32) {
+          const int t = atomicAdd(&s_cnt, 1);
+          s_nb[t] = i0 + u * T;
+          s_nbw[t] = wv[u];
         }
       }
     }
@@ -352,24 +370,36 @@ __global__ void __launch_bounds__(1024) extract_bags_kernel(BagArgs a) {
     const int64_t out = a.bag_off[gv];
     const int32_t* Wi = Wp + (int64_t)i * np;
     int outp = 0;
-    for (int c = 0; c < np; c += 64) {
-      const int j = c + lane;
-      bool keep = false;
-      int32_t w = 0, gj = 0;
-      if (j < np && j != i) {
-        w = Wi[j];
-        if (w != INF32) {
-          gj = a.members[base + j];
-          keep = (a.outcnt[base + j] != 0) || (a.elim_pos[gj] > pv);
+    // U chunks of 64 columns per pass, their loads in flight together
+    constexpr int U = 4;
+    for (int c0 = 0; c0 < np; c0 += 64 * U) {
+      int32_t w[U], gj[U];
+      bool keep[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = c0 + u * 64 + lane;
+        w[u] = (j < np && j != i) ? Wi[j] : INF32;
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const int j = c0 + u * 64 + lane;
+        keep[u] = false;
+        gj[u] = 0;
+        if (w[u] != INF32) {
+          gj[u] = a.members[base + j];
+          keep[u] = (a.outcnt[base + j] != 0) || (a.elim_pos[gj[u]] > pv);
         }
       }
-      const uint64_t m = __ballot(keep);
-      if (keep) {
-        const int pos = outp + __popcll(m & lt);
-        a.bag_nbr[out + pos] = gj;
-        a.bag_w[out + pos] = w;
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        const uint64_t m = __ballot(keep[u]);
+        if (keep[u]) {
+          const int pos = outp + __popcll(m & lt);
+          a.bag_nbr[out + pos] = gj[u];
+          a.bag_w[out + pos] = w[u];
+        }
+        outp += __popcll(m);
       }
-      outp += __popcll(m);
     }
   }
 }
@@ -503,11 +533,13 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     a.elim_pos = d_epos.as<int32_t>();
     a.bag_len = d_blen.as<int32_t>();
     a.gstate = D.gstate.as<int32_t>();
-    a.state_in_lds = D.lds ? 1 : 0;
     a.stale_first = (gpu_order && P.height > 0) ? 1 : 0;
     a.track_holes = (gpu_order && P.height > 0) ? 1 : 0;
     a.err = d_err.as<int32_t>();
-    hipLaunchKernelGGL(contract_kernel, dim3((unsigned)P.heap.size()), dim3(D.block), D.shmem, s, a);
+    if (D.lds)
+      hipLaunchKernelGGL(contract_kernel<true>, dim3((unsigned)P.heap.size()), dim3(D.block), D.shmem, s, a);
+    else
+      hipLaunchKernelGGL(contract_kernel<false>, dim3((unsigned)P.heap.size()), dim3(D.block), D.shmem, s, a);
     H2H_HIP(ctx, hipGetLastError());
   }
   host_lap("  contraction: levels launched");
