@@ -332,7 +332,9 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
     return H2H_OK;
   };
   // coordinate descent from the defaults: each parameter's alternatives with
-  // the others held at the best so far
+  // the others held at the best so far; a candidate replaces the best only
+  // when it is at least 3% faster, so that replay noise cannot make the
+  // choice differ from run to run
   BatchedConfig cur;
   double cur_us = 0;
   h2h_status st = measure(cur, cur_us);
@@ -345,7 +347,7 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
       double us = 0;
       h2h_status s2 = measure(c, us);
       if (s2) return s2;
-      if (us < cur_us) {
+      if (us < 0.97 * cur_us) {
         cur = c;
         cur_us = us;
       }

@@ -235,7 +235,10 @@ def test_tune_batched_keeps_labels():
         o, p = _contract_both(case, C, h2h.CONTRACT_GPU_ORDER, threads=4)
         with p:
             best, tried = p.tune_batched(h2h.L4, replays=2)
-            assert len(tried) >= 5 and best["build_us"] == min(t["build_us"] for t in tried)
+            # the kept configuration was measured, and no candidate beat it by
+            # the tuner's 3% acceptance margin
+            assert len(tried) >= 5 and best in tried
+            assert all(t["build_us"] >= 0.97 * best["build_us"] for t in tried)
             o.labels(O.SERIAL, l4=True)
             np.testing.assert_array_equal(p.labels()["dis"], o.dis())
             p.build_labels(h2h.L4, h2h.BATCHED)
