@@ -1,0 +1,238 @@
+// Index persistence: one binary file with what answering queries and
+// rebuilding labels need -- the tree (father, depth), the bags with their
+// two-hop marks, and the labels row by row in vertex order -- so that a
+// serving process restarts without contracting or building.  The reference
+// keeps its index in memory only.
+//
+// File layout (little-endian, as written by this library):
+//   Header (64 bytes), then
+//   father[n] int32, depth[n] int32,
+//   bag_off[n+1] int64, bag_nbr[nb] int32, bag_w[nb] int32,
+//   bag_dom[nb] uint8 (only when has_dom),
+//   the label rows (only when has_labels): for v = 0..n-1, depth[v] int32
+//   (dis[v][0..depth-1]),
+//   then a 64-bit FNV-1a-style hash of the byte stream after the header,
+//   taken 8 bytes at a time (a final partial word byte by byte).
+// Loading rebuilds the tree layout from the bags (father = deepest bag
+// member, as for h2h_labels_upload) and scatters the rows into it, so the
+// file does not depend on how the saving process laid its arrays out.  A
+// file without labels (a few percent of the size) has them rebuilt on the
+// device by the batched build, which takes milliseconds where reading the
+// rows takes a large part of a second per GB.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+
+#include "h2h_internal.h"
+
+using namespace h2h;
+
+namespace {
+
+constexpr char MAGIC[8] = {'H', '2', 'H', 'I', 'D', 'X', '0', '1'};
+constexpr int32_t VERSION = 1;
+
+struct Header {
+  char magic[8];
+  int32_t version;
+  int32_t label_kind;
+  int64_t n;
+  int64_t nb;             // bag entries
+  int64_t label_entries;  // sum of depth
+  int32_t has_dom;
+  int32_t has_labels;
+  int32_t reserved[4];
+};
+static_assert(sizeof(Header) == 64, "header layout");
+
+// streaming: the result does not depend on how the stream is cut into calls
+struct Fnv {
+  static constexpr uint64_t PRIME = 1099511628211ull;
+  uint64_t h = 1469598103934665603ull;
+  unsigned char pend[8];
+  size_t np = 0;
+  void add(const void* p, size_t n) {
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    size_t i = 0;
+    while (np && i < n) {
+      pend[np++] = b[i++];
+      if (np == 8) {
+        word(pend);
+        np = 0;
+      }
+    }
+    for (; i + 8 <= n; i += 8) word(b + i);
+    while (i < n) pend[np++] = b[i++];
+  }
+  void word(const unsigned char* b) {
+    uint64_t w;
+    std::memcpy(&w, b, 8);
+    h = (h ^ w) * PRIME;
+  }
+  uint64_t value() const {
+    uint64_t x = h;
+    for (size_t k = 0; k < np; k++) x = (x ^ pend[k]) * PRIME;
+    return x;
+  }
+};
+
+struct Writer {
+  std::ofstream f;
+  Fnv fnv;
+  bool put(const void* p, size_t n) {
+    fnv.add(p, n);
+    f.write(static_cast<const char*>(p), (std::streamsize)n);
+    return (bool)f;
+  }
+};
+
+struct Reader {
+  std::ifstream f;
+  Fnv fnv;
+  bool get(void* p, size_t n) {
+    f.read(static_cast<char*>(p), (std::streamsize)n);
+    if (!f || (size_t)f.gcount() != n) return false;
+    fnv.add(p, n);
+    return true;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+h2h_status h2h_index_save(h2h_ctx* ctx, const char* path, int with_labels) {
+  if (!ctx || !path) return H2H_ERR_ARG;
+  if (!ctx->labels_ready) return set_err(ctx, H2H_ERR_STATE, "build (or load) labels before saving the index");
+  const int64_t n = ctx->n, nb = ctx->bag_off[n], L = ctx->label_entries;
+  std::vector<int32_t> dis;
+  if (with_labels) {
+    dis.resize(std::max<int64_t>(L, 1));
+    (void)hipSetDevice(ctx->device);
+    H2H_HIP(ctx, hipMemcpyAsync(dis.data(), ctx->d_dis.p, L * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    H2H_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  Writer w;
+  w.f.open(path, std::ios::binary | std::ios::trunc);
+  if (!w.f.is_open()) return set_err(ctx, H2H_ERR_IO, std::string("cannot write ") + path);
+  Header h{};
+  std::memcpy(h.magic, MAGIC, 8);
+  h.version = VERSION;
+  h.label_kind = ctx->label_kind;
+  h.n = n;
+  h.nb = nb;
+  h.label_entries = L;
+  h.has_dom = ctx->bag_dom.empty() ? 0 : 1;
+  h.has_labels = with_labels ? 1 : 0;
+  w.f.write(reinterpret_cast<const char*>(&h), sizeof(h));
+  bool ok = (bool)w.f;
+  ok = ok && w.put(ctx->father.data(), n * sizeof(int32_t)) && w.put(ctx->depth.data(), n * sizeof(int32_t));
+  ok = ok && w.put(ctx->bag_off.data(), (n + 1) * sizeof(int64_t)) && w.put(ctx->bag_nbr.data(), nb * sizeof(int32_t)) &&
+       w.put(ctx->bag_w.data(), nb * sizeof(int32_t));
+  if (h.has_dom) ok = ok && w.put(ctx->bag_dom.data(), nb);
+  for (int64_t v = 0; ok && with_labels && v < n; v++)
+    ok = w.put(dis.data() + ctx->dis_hash[v], ctx->depth[v] * sizeof(int32_t));
+  const uint64_t sum = w.fnv.value();
+  ok = ok && (bool)w.f.write(reinterpret_cast<const char*>(&sum), sizeof(sum));
+  w.f.close();
+  if (!ok || !w.f) return set_err(ctx, H2H_ERR_IO, std::string("writing ") + path + " failed");
+  return H2H_OK;
+}
+
+h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
+  if (!ctx || !path) return H2H_ERR_ARG;
+  if (ctx->device < 0) return set_err(ctx, H2H_ERR_NOGPU, "host-only context (created with device -1)");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the index");
+  Reader r;
+  r.f.open(path, std::ios::binary);
+  if (!r.f.is_open()) return set_err(ctx, H2H_ERR_IO, std::string("cannot read ") + path);
+  Header h{};
+  r.f.read(reinterpret_cast<char*>(&h), sizeof(h));
+  if (!r.f || std::memcmp(h.magic, MAGIC, 8) != 0) return set_err(ctx, H2H_ERR_IO, "not an h2h index file");
+  if (h.version != VERSION) return set_err(ctx, H2H_ERR_IO, "unsupported index file version");
+  if (h.n < 1 || h.n > INT32_MAX - 1 || h.nb < 0 || h.label_entries < h.n ||
+      (h.label_kind != H2H_LABEL_L3 && h.label_kind != H2H_LABEL_L4) || (h.has_dom | h.has_labels) > 1 ||
+      h.has_dom < 0 || h.has_labels < 0)
+    return set_err(ctx, H2H_ERR_IO, "index file header out of range");
+  // sizes fixed by the header must fit in what the file holds
+  r.f.seekg(0, std::ios::end);
+  const int64_t fsize = (int64_t)r.f.tellg();
+  const int64_t need = (int64_t)sizeof(Header) + 8 * h.n + 8 * (h.n + 1) + 8 * h.nb + (h.has_dom ? h.nb : 0) +
+                       (h.has_labels ? 4 * h.label_entries : 0) + 8;
+  if (fsize != need) return set_err(ctx, H2H_ERR_IO, "index file size does not match its header");
+  r.f.seekg((std::streamoff)sizeof(Header), std::ios::beg);
+  const int n = (int)h.n;
+  std::vector<int32_t> father(n), depth(n);
+  std::vector<int64_t> bag_off(n + 1);
+  std::vector<int32_t> bag_nbr(h.nb), bag_w(h.nb), rows(h.has_labels ? h.label_entries : 0);
+  std::vector<uint8_t> bag_dom(h.has_dom ? h.nb : 0);
+  bool ok = r.get(father.data(), n * sizeof(int32_t)) && r.get(depth.data(), n * sizeof(int32_t)) &&
+            r.get(bag_off.data(), (n + 1) * sizeof(int64_t)) && r.get(bag_nbr.data(), h.nb * sizeof(int32_t)) &&
+            r.get(bag_w.data(), h.nb * sizeof(int32_t));
+  if (ok && h.has_dom) ok = r.get(bag_dom.data(), h.nb);
+  if (h.has_labels) ok = ok && r.get(rows.data(), h.label_entries * sizeof(int32_t));
+  const uint64_t sum_read = r.fnv.value();
+  uint64_t sum = 0;
+  r.f.read(reinterpret_cast<char*>(&sum), sizeof(sum));
+  if (!ok || !r.f || sum != sum_read) return set_err(ctx, H2H_ERR_IO, "index file is truncated or corrupt (checksum)");
+  // structural checks before anything of the context is replaced
+  // (a tree of depths, bag members strictly above their vertex, so that the
+  // rebuilt tree cannot hold a cycle)
+  int64_t tot = 0;
+  int roots = 0;
+  if (bag_off[0] != 0 || bag_off[n] != h.nb) return set_err(ctx, H2H_ERR_IO, "bag offsets out of range");
+  for (int v = 0; v < n; v++) {
+    if (depth[v] < 1 || bag_off[v + 1] < bag_off[v]) return set_err(ctx, H2H_ERR_IO, "bad depth or bag offsets");
+    if (father[v] < 0) roots += (depth[v] == 1) ? 1 : 2;
+    else if (father[v] >= n || depth[father[v]] + 1 != depth[v]) return set_err(ctx, H2H_ERR_IO, "father/depth inconsistent");
+    tot += depth[v];
+    for (int64_t e = bag_off[v]; e < bag_off[v + 1]; e++)
+      if (bag_nbr[e] < 0 || bag_nbr[e] >= n || depth[bag_nbr[e]] >= depth[v])
+        return set_err(ctx, H2H_ERR_IO, "bag member out of range or not above its vertex");
+  }
+  if (roots != 1) return set_err(ctx, H2H_ERR_IO, "the saved tree needs exactly one root of depth 1");
+  if (tot != h.label_entries) return set_err(ctx, H2H_ERR_IO, "label rows do not match the depths");
+
+  (void)hipSetDevice(ctx->device);
+  ctx->tree_ready = ctx->labels_ready = ctx->contracted = false;
+  invalidate_label_plans(ctx);
+  ctx->n = n;
+  ctx->bag_off = std::move(bag_off);
+  ctx->bag_nbr = std::move(bag_nbr);
+  ctx->bag_w = std::move(bag_w);
+  ctx->bag_dom = std::move(bag_dom);
+  // an elimination record whose rank order gives father = deepest bag member
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(),
+            [&](int32_t a, int32_t b) { return depth[a] != depth[b] ? depth[a] > depth[b] : a < b; });
+  ctx->elim = order;
+  ctx->rec_deg.assign(n, 0);
+  ctx->rec_height.assign(n, 0);
+  h2h_status st = build_tree_and_layout(ctx);
+  if (st) return st;
+  for (int v = 0; v < n; v++)
+    if (ctx->father[v] != father[v] || ctx->depth[v] != depth[v]) {
+      ctx->tree_ready = false;
+      return set_err(ctx, H2H_ERR_IO, "the bags do not give the saved tree");
+    }
+  if ((st = upload_index(ctx))) return st;
+  if (!h.has_labels) return run_label_build(ctx, h.label_kind, H2H_METHOD_BATCHED, nullptr);
+  // the saved rows, scattered into this layout
+  std::vector<int32_t> dis(h.label_entries);
+  int64_t k = 0;
+  for (int v = 0; v < n; v++) {
+    std::copy(rows.begin() + k, rows.begin() + k + depth[v], dis.begin() + ctx->dis_hash[v]);
+    k += depth[v];
+  }
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->d_dis.p, dis.data(), h.label_entries * sizeof(int32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+  H2H_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->label_kind = h.label_kind;
+  ctx->labels_ready = true;
+  return H2H_OK;
+}
+
+}  // extern "C"

@@ -9,7 +9,10 @@
 // (main.cu:149-240) and appends the reference's CSV rows:
 //   ./ConstructInfo/<G>/contract.csv and constructInfo.csv (31 columns),
 //   ./QueryResult/<G>/random.csv (23 columns),
-//   ./<Method>PartitionRate.csv.
+//   ./<Method>PartitionRate.csv,
+// and, for GraphName NY with ./Query/NY/{normal,sparse}.txt present, the
+// real-query row (answerRealNYQuery_noHub) in
+//   ./QueryResult/NY/PTHeight-<H>-ChangeHeight-<C>-real.csv (19 columns).
 // Every phase runs on the GPU: useGPUContract picks the elimination order
 // (0 = partition_TD_T's, 1 = makeCHPerHeight_D_2's), parallelFine picks the
 // build's work split (0/1 per node, 2 per bag entry, 3 batched), and
@@ -17,6 +20,7 @@
 // H2H_LABEL_KIND=3 reproduces the as-shipped L3 labels (default 4, exact).
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -49,6 +53,62 @@ int check(h2h_ctx* ctx, h2h_status st, const char* what) {
     std::fprintf(stderr, "%s failed (%d): %s\n", what, st, ctx ? h2h_last_error(ctx) : "");
     return 1;
   }
+  return 0;
+}
+
+// Up to `limit` (s, t, other) triples of a query file; fewer when it ends
+// early (uploadRealQuery, Query.cu:21-44, reads 1e5 from each file)
+void read_triples(const std::string& path, int64_t limit, std::vector<int32_t>& x, std::vector<int32_t>& y) {
+  std::ifstream f(path);
+  long long s, t, other;
+  for (int64_t i = 0; i < limit && (f >> s >> t >> other); i++) {
+    x.push_back((int32_t)s);
+    y.push_back((int32_t)t);
+  }
+}
+
+// answerRealNYQuery_noHub (Query.cu:626-760): the queries of
+// ./Query/NY/normal.txt and ./Query/NY/sparse.txt (vertex ids as written),
+// timed on the plain kernels, one row appended to
+// ./QueryResult/NY/PTHeight-<H>-ChangeHeight-<C>-real.csv under the
+// reference's 19-column header.  Skipped when either file is missing.
+int real_queries(h2h_ctx* ctx, const h2h_info& info, int C, int label_kind) {
+  const std::string normal = "./Query/NY/normal.txt", sparse = "./Query/NY/sparse.txt";
+  if (!file_exists(normal) || !file_exists(sparse)) return 0;
+  std::vector<int32_t> x, y;
+  read_triples(normal, 100000, x, y);
+  read_triples(sparse, 100000, x, y);
+  const int64_t nq = (int64_t)x.size();
+  for (int64_t i = 0; i < nq; i++)
+    if (x[i] < 0 || x[i] >= info.n || y[i] < 0 || y[i] >= info.n) {
+      std::fprintf(stderr, "real queries: pair %lld (%d, %d) is outside the graph's %lld vertices\n", (long long)i,
+                   x[i], y[i], (long long)info.n);
+      return 1;
+    }
+  std::vector<int32_t> out(std::max<int64_t>(nq, 1));
+  double t_nolca = 0, t_lca = 0;
+  if (nq > 0) {
+    if (check(ctx, h2h_query(ctx, H2H_QUERY_NOLCA, H2H_EXEC_PLAIN, x.data(), y.data(), out.data(), nq, &t_nolca),
+              "real queries"))
+      return 1;
+    if (label_kind == H2H_LABEL_L4 &&
+        check(ctx, h2h_query(ctx, H2H_QUERY_LCA, H2H_EXEC_PLAIN, x.data(), y.data(), out.data(), nq, &t_lca),
+              "real queries"))
+      return 1;
+  }
+  std::printf("real queries (%lld): noLCA %.0f us, LCA %.0f us\n", (long long)nq, t_nolca, t_lca);
+  mkdirs("./QueryResult/NY");
+  std::ofstream q("./QueryResult/NY/PTHeight-" + std::to_string(info.partition_height) + "-ChangeHeight-" +
+                      std::to_string(C) + "-real.csv",
+                  std::ios::app);
+  // the reference writes the header before every row, and 10000 as the
+  // query number (its loop starts at i = queryMaxSize but prints 10000 + 0 * queryStep)
+  q << "queryNumber,sequence,multiThread-4,multiThread-6,multiThread-8,multiThread-10,multiThread-12,"
+       "multiThread-14,multiThread-16,GPUnoLCA-64,GPUwithLCA-64,GPUnoLCA-128,GPUwithLCA-128,GPUnoLCA-256,"
+       "GPUwithLCA-256,GPUnoLCA-512,GPUwithLCA-512,GPUnoLCA-1024,GPUwithLCA-1024\n";
+  q << 10000 << ",-1,-1,-1,-1,-1,-1,-1,-1";
+  for (int b = 0; b < 5; b++) q << "," << (long long)t_nolca << "," << (long long)t_lca;
+  q << "\n";
   return 0;
 }
 
@@ -163,6 +223,8 @@ int main(int argc, char** argv) {
   q << nq << ",-1,-1,-1,-1,-1,-1,-1,-1";
   for (int b = 0; b < 5; b++) q << "," << (long long)t_nolca << "," << (long long)t_lca;
   q << "," << info.partition_height << "," << C << "," << method << "," << contract_device << "\n";
+  q.close();
+  const int rc = graph == "NY" ? real_queries(ctx, info, C, label_kind) : 0;
   h2h_destroy(ctx);
-  return 0;
+  return rc;
 }

@@ -87,6 +87,8 @@ def lib():
             "h2h_build_labels_profiled": [P, ctypes.c_int, ctypes.c_int, dp, i64p],
             "h2h_tune_batched": [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(TuneResult),
                                  ctypes.POINTER(TuneResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+            "h2h_index_save": [P, ctypes.c_char_p, ctypes.c_int],
+            "h2h_index_load": [P, ctypes.c_char_p],
             "h2h_get_batched_config": [P, ctypes.POINTER(TuneResult)],
             "h2h_set_batched_config": [P, ctypes.POINTER(TuneResult)],
             "h2h_labels_upload": [P, ctypes.c_int32, i64p, i32p, i64p, i32p, i32p, i32p, i32p],
@@ -206,6 +208,16 @@ class H2HIndex:
         self._chk(lib().h2h_tune_batched(self.h, int(kind), int(replays), ctypes.byref(best), tried, 32,
                                          ctypes.byref(nt)))
         return best.as_dict(), [tried[i].as_dict() for i in range(min(nt.value, 32))]
+
+    def save(self, path, with_labels=True):
+        """Writes the built index to `path` (h2h_index_save); without the
+        labels the file is small and loading rebuilds them on the device."""
+        self._chk(lib().h2h_index_save(self.h, os.fsencode(path), int(bool(with_labels))))
+
+    def load(self, path):
+        """Replaces this context's index with the one saved at `path`
+        (h2h_index_load): queries and serving work at once, no build."""
+        self._chk(lib().h2h_index_load(self.h, os.fsencode(path)))
 
     def batched_config(self):
         """The context's batched-build parameters (h2h_get_batched_config)."""

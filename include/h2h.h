@@ -146,6 +146,18 @@ typedef struct {
 h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_result* best, h2h_tune_result* tried,
                             int max_tried, int* ntried);
 
+/* Index persistence: writes the built index (tree, bags with their two-hop
+ * marks, and the labels when with_labels is nonzero) to one checksummed
+ * binary file, and loads such a file into a context, which can then answer
+ * queries, serve and rebuild labels without contracting.  A file without
+ * labels is a few percent of the size; loading it rebuilds the labels (of
+ * the saved kind) with the batched build.  Loading checks the header, the
+ * sizes, the checksum and that the bags give the saved tree (H2H_ERR_IO
+ * otherwise); on an error after the checks the context holds no index.  No
+ * counterpart in the reference, which keeps its index in memory only. */
+h2h_status h2h_index_save(h2h_ctx* ctx, const char* path, int with_labels);
+h2h_status h2h_index_load(h2h_ctx* ctx, const char* path);
+
 /* Reads / sets the context's batched-build parameters (what h2h_tune_batched
  * chose, or one saved from an earlier tuning; build_us is ignored).  Setting
  * them drops the batched plan, so the next H2H_METHOD_BATCHED build plans

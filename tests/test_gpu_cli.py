@@ -34,6 +34,33 @@ def test_g2h_runs_and_writes_csvs(tmp_path, args):
     assert (tmp_path / "ConstructInfo/13/contract.csv").exists()
 
 
+def test_g2h_real_queries(tmp_path):
+    """GraphName NY with ./Query/NY/{normal,sparse}.txt present: the real
+    queries (answerRealNYQuery_noHub, Query.cu:21-44, 626-760) are read as
+    (s, t, other) triples, answered, and one row goes under the 19-column
+    header of PTHeight-<H>-ChangeHeight-<C>-real.csv; a pair outside the
+    graph is an error."""
+    os.makedirs(tmp_path / "Graph/USA")
+    os.makedirs(tmp_path / "Partition/NY")
+    os.makedirs(tmp_path / "Query/NY")
+    for f in ("Edge.txt", "Node.txt"):
+        shutil.copy(os.path.join(cases.REF13, "13" + f), tmp_path / "Graph/USA" / ("NY" + f))
+    for suf in ("ID.txt", "Cut.txt"):
+        shutil.copy(os.path.join(cases.REF13, "Latitude" + suf), tmp_path / "Partition/NY" / ("Latitude" + suf))
+    (tmp_path / "Query/NY/normal.txt").write_text("".join(f"{i % 13} {(5 * i) % 13} 0\n" for i in range(40)))
+    (tmp_path / "Query/NY/sparse.txt").write_text("".join(f"{(3 * i) % 13} {i % 13} 7\n" for i in range(25)))
+    args = ["NY", "4", "1", "4", "1", "1", "3", "1"]
+    r = subprocess.run([G2H] + args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "real queries (65)" in r.stdout
+    rows = (tmp_path / "QueryResult/NY/PTHeight-4-ChangeHeight-1-real.csv").read_text().splitlines()
+    assert len(rows) == 2 and rows[0].startswith("queryNumber,sequence,") and len(rows[0].split(",")) == 19
+    assert rows[1].split(",")[0] == "10000" and len(rows[1].split(",")) == 19
+    (tmp_path / "Query/NY/sparse.txt").write_text("1 13 0\n")
+    r = subprocess.run([G2H] + args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "outside the graph" in r.stderr
+
+
 def test_g2h_wrong_argc_exits_zero(tmp_path):
     r = subprocess.run([G2H, "13"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 0

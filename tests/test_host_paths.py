@@ -135,10 +135,21 @@ def test_device_calls_on_host_context(tmp_path):
     with _host() as p:
         p.set_graph(3, u, v, w, np.arange(3), _cut1(3), 1)
         for call in (lambda: p.contract(1), lambda: p.build_labels(), lambda: p.query([0], [1]),
-                     lambda: p.server_start(), lambda: p.tune_batched()):
+                     lambda: p.server_start(), lambda: p.tune_batched(),
+                     lambda: p.load(str(tmp_path / "none.h2h"))):
             with pytest.raises(h2h.H2HError) as e:
                 call()
             assert e.value.status == -8
+
+
+def test_save_needs_labels(tmp_path):
+    u, v, w = _tri()
+    with _host() as p:
+        p.set_graph(3, u, v, w, np.arange(3), _cut1(3), 1)
+        with pytest.raises(h2h.H2HError) as e:
+            p.save(str(tmp_path / "x.h2h"))
+        assert e.value.status == -7
+    assert not (tmp_path / "x.h2h").exists()
 
 
 def test_batched_config_round_trip():
