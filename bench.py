@@ -428,6 +428,19 @@ def main():
     rate = 0.5 * 4096 / (np.median(lat_direct) / 1000.0)
     lat_served = served_latency(p, g.n, rate, 400, 4096, 7)
 
+    # ---- shortest-path retrieval (paths, not only distances) ----
+    npath = 4096
+    paths, path_us = p.shortest_paths(x[:npath], y[:npath])
+    hops = np.array([len(P) - 1 for P in paths])
+    start = np.searchsorted(u, np.arange(g.n + 1))   # arcs are grouped by source, targets ascending
+    for i in range(len(src)):                        # the Dijkstra rows of the spot check above
+        P = paths[i].astype(np.int64)
+        idx = [start[a] + np.searchsorted(v[start[a]:start[a + 1]], b) for a, b in zip(P[:-1], P[1:])]
+        if P[0] != x[i] or P[-1] != y[i] or not np.array_equal(v[idx], P[1:]) or w[idx].sum() != D[i][y[i]]:
+            raise SystemExit("a retrieved path is not a shortest path")
+    path_stats = {"pairs": npath, "wall_s": path_us * 1e-6, "paths_per_s": npath / (path_us * 1e-6),
+                  "avg_hops": float(hops.mean()), "max_hops": int(hops.max())}
+
     # ---- index persistence: save, then load into a fresh context (what a
     # restarted server does instead of contracting and building) ----
     import tempfile
@@ -504,6 +517,7 @@ def main():
         "query_roofline": q_roof,
         "batched_over_plain_lca": q["lca_batched_qps"] / q["lca_plain_qps"],
         "index_persistence": persist,
+        "shortest_paths": path_stats,
         "latency_4k_ms": {"direct_p50": float(np.percentile(lat_direct, 50)),
                           "direct_p99": float(np.percentile(lat_direct, 99)),
                           "served_p50": float(np.percentile(lat_served, 50)),

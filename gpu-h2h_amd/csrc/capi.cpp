@@ -439,6 +439,7 @@ h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, c
   if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the index");
   (void)hipSetDevice(ctx->device);
   ctx->tree_ready = ctx->labels_ready = ctx->contracted = false;
+  ctx->graph_loaded = false;  // the index no longer belongs to a loaded graph
   invalidate_label_plans(ctx);
   ctx->n = n;
   // bags from pos (self excluded), weights from the initial dis rows

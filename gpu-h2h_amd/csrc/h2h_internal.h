@@ -205,6 +205,10 @@ struct h2h_ctx {
   // ---- timings ----
   double t_partition_us = 0, t_contract_us = 0, t_tree_us = 0, t_label_us = 0;
 
+  // ---- shortest paths of the last h2h_shortest_paths call (CSR) ----
+  std::vector<int64_t> path_off;
+  std::vector<int32_t> path_vertices;
+
   // ---- serving ----
   std::unique_ptr<h2h::Server> server;
 

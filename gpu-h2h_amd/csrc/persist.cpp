@@ -197,6 +197,7 @@ h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
 
   (void)hipSetDevice(ctx->device);
   ctx->tree_ready = ctx->labels_ready = ctx->contracted = false;
+  ctx->graph_loaded = false;  // the index no longer belongs to a loaded graph
   invalidate_label_plans(ctx);
   ctx->n = n;
   ctx->bag_off = std::move(bag_off);

@@ -88,6 +88,8 @@ def lib():
             "h2h_tune_batched": [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(TuneResult),
                                  ctypes.POINTER(TuneResult), ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
             "h2h_index_save": [P, ctypes.c_char_p, ctypes.c_int],
+            "h2h_shortest_paths": [P, i32p, i32p, ctypes.c_int64, i64p, dp],
+            "h2h_get_paths": [P, i64p, i32p],
             "h2h_index_load": [P, ctypes.c_char_p],
             "h2h_get_batched_config": [P, ctypes.POINTER(TuneResult)],
             "h2h_set_batched_config": [P, ctypes.POINTER(TuneResult)],
@@ -208,6 +210,19 @@ class H2HIndex:
         self._chk(lib().h2h_tune_batched(self.h, int(kind), int(replays), ctypes.byref(best), tried, 32,
                                          ctypes.byref(nt)))
         return best.as_dict(), [tried[i].as_dict() for i in range(min(nt.value, 32))]
+
+    def shortest_paths(self, s, t):
+        """Shortest paths for the pairs (s[i], t[i]) (h2h_shortest_paths):
+        (list of vertex arrays, host microseconds)."""
+        s = np.ascontiguousarray(s, np.int32)
+        t = np.ascontiguousarray(t, np.int32)
+        tot = ctypes.c_int64()
+        us = ctypes.c_double()
+        self._chk(lib().h2h_shortest_paths(self.h, _p32(s), _p32(t), len(s), ctypes.byref(tot), ctypes.byref(us)))
+        off = np.empty(len(s) + 1, np.int64)
+        verts = np.empty(max(tot.value, 1), np.int32)
+        self._chk(lib().h2h_get_paths(self.h, _p64(off), _p32(verts)))
+        return [verts[off[i]:off[i + 1]] for i in range(len(s))], us.value
 
     def save(self, path, with_labels=True):
         """Writes the built index to `path` (h2h_index_save); without the

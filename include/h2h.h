@@ -158,6 +158,19 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
 h2h_status h2h_index_save(h2h_ctx* ctx, const char* path, int with_labels);
 h2h_status h2h_index_load(h2h_ctx* ctx, const char* path);
 
+/* Shortest paths (not only distances) for npairs pairs (s[i], t[i]): each
+ * path is walked from s over neighbours n with w(c, n) + d(n, t) = d(c, t),
+ * one batched distance query per hop for all pairs at once, so it needs
+ * the graph of the index (h2h_set_graph or h2h_load_files, not an uploaded
+ * or loaded index: H2H_ERR_STATE).  *total_vertices gets the number of
+ * vertices over all paths (s and t included; a path from s to s is [s]);
+ * h2h_get_paths then copies path_off[npairs + 1] and the vertices out.
+ * The reference's hub labels (H2HLabel::Hub, Graph_D.cuh:57-84) record the
+ * bag member behind each label entry but unpack no path. */
+h2h_status h2h_shortest_paths(h2h_ctx* ctx, const int32_t* s, const int32_t* t, int64_t npairs,
+                              int64_t* total_vertices, double* usec);
+h2h_status h2h_get_paths(h2h_ctx* ctx, int64_t* path_off, int32_t* vertices);
+
 /* Reads / sets the context's batched-build parameters (what h2h_tune_batched
  * chose, or one saved from an earlier tuning; build_us is ignored).  Setting
  * them drops the batched plan, so the next H2H_METHOD_BATCHED build plans

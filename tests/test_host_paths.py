@@ -136,7 +136,8 @@ def test_device_calls_on_host_context(tmp_path):
         p.set_graph(3, u, v, w, np.arange(3), _cut1(3), 1)
         for call in (lambda: p.contract(1), lambda: p.build_labels(), lambda: p.query([0], [1]),
                      lambda: p.server_start(), lambda: p.tune_batched(),
-                     lambda: p.load(str(tmp_path / "none.h2h"))):
+                     lambda: p.load(str(tmp_path / "none.h2h")),
+                     lambda: p.shortest_paths(np.array([0], np.int32), np.array([1], np.int32))):
             with pytest.raises(h2h.H2HError) as e:
                 call()
             assert e.value.status == -8
