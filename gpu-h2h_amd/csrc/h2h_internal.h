@@ -245,5 +245,11 @@ h2h_status profile_label_build(h2h_ctx* ctx, int kind, int method, double* kerne
 h2h_status build_lca(h2h_ctx* ctx);                                    // query.hip
 h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, const int32_t* dy,
                         int32_t* dout, int64_t n, hipStream_t s);  // query.hip
+// shortest-path walk, one wavefront per pair (query.hip): off == nullptr
+// counts each path's vertices into len (-1: the host must walk it), else
+// writes them at off
+h2h_status launch_path_walk(h2h_ctx* ctx, int kind, const int64_t* adj_off, const int32_t* adj_nbr,
+                            const int32_t* adj_w, const int32_t* ds, const int32_t* dt, const int32_t* d0,
+                            const int64_t* off, int32_t* verts, int32_t* len, int64_t npairs, hipStream_t s);
 
 }  // namespace h2h

@@ -214,6 +214,87 @@ __global__ void __launch_bounds__(256) query_tile_kernel(QArgs q, const int32_t*
   tile_pass<4>(q, kind, meta, list[0], cnt[0], base, out);
 }
 
+// ---- shortest-path walk ----
+
+// d(a, b) by one wavefront: the LCA from every lane (uniform loads), the
+// label scan over 64 lanes, then a wavefront minimum
+__device__ inline int32_t wave_dist(const QArgs& q, int kind, int a, int b, int lane) {
+  if (a == b) return 0;
+  const Lca L = find_lca(q, a, b);
+  const int32_t* dx = q.dis + q.dis_hash[a];
+  const int32_t* dy = q.dis + q.dis_hash[b];
+  int32_t r = INF32;
+  if (kind == H2H_QUERY_LCA) {
+    const int64_t e1 = q.pos_hash[L.node + 1];
+    for (int64_t e = q.pos_hash[L.node] + lane; e < e1; e += 64) {
+      const int p = q.pos_pos[e];
+      r = min(r, dx[p] + dy[p]);
+    }
+  } else {
+    for (int j = lane; j < L.depth; j += 64) r = min(r, dx[j] + dy[j]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) r = min(r, __shfl_xor(r, off, 64));
+  return r;
+}
+
+struct WalkArgs {
+  const int64_t* adj_off;
+  const int32_t* adj_nbr;
+  const int32_t* adj_w;
+  const int32_t* s;
+  const int32_t* t;
+  const int32_t* d0;     // d(s, t)
+  const int64_t* off;    // second pass: where each path's vertices go
+  int32_t* verts;        // second pass: the vertices
+  int32_t* len;          // vertices per path, -1 when the walk needs the host (written by the
+                         // first pass; the second pass skips the -1 pairs, which own no slots)
+  int64_t npairs;
+  int kind;
+};
+
+// One wavefront per pair: from the current vertex c, the next vertex is the
+// neighbour n with w(c, n) > 0 and w(c, n) + d(n, t) = d(c, t) of least
+// (d(n, t), adjacency position).  d strictly falls, so the walk ends at t
+// within d(s, t) hops.  A vertex whose shortest continuations all use
+// zero-weight edges ends the walk with length -1 (the host walks those
+// pairs, stepping back where needed).  The first pass (off == nullptr)
+// counts the vertices, the second writes them.
+__global__ void __launch_bounds__(256) path_walk_kernel(QArgs q, WalkArgs w) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= w.npairs) return;  // whole wavefronts
+  if (w.off && w.len[i] < 0) return;
+  const int tt = w.t[i];
+  int cur = w.s[i];
+  int32_t dc = w.d0[i];
+  int64_t k = 0;
+  if (w.off && lane == 0) w.verts[w.off[i]] = cur;
+  while (cur != tt) {
+    const int64_t e0 = w.adj_off[cur], e1 = w.adj_off[cur + 1];
+    uint64_t best = ~0ull;
+    for (int64_t e = e0; e < e1; e++) {
+      const int32_t wt = w.adj_w[e];
+      if (wt <= 0) continue;
+      const int nb = w.adj_nbr[e];
+      const int32_t dn = wave_dist(q, w.kind, nb, tt, lane);
+      if (dn != INF32 && (int64_t)wt + dn == (int64_t)dc) {
+        const uint64_t key = ((uint64_t)(uint32_t)dn << 32) | (uint64_t)(e - e0);
+        best = key < best ? key : best;
+      }
+    }
+    if (best == ~0ull) {  // only zero-weight continuations (or none)
+      if (!w.off && lane == 0) w.len[i] = -1;
+      return;
+    }
+    cur = w.adj_nbr[e0 + (int64_t)(best & 0xffffffffu)];
+    dc = (int32_t)(best >> 32);
+    k++;
+    if (w.off && lane == 0) w.verts[w.off[i] + k] = cur;
+  }
+  if (!w.off && lane == 0) w.len[i] = (int32_t)(k + 1);
+}
+
 }  // namespace
 
 #define H2H_TRY(x)               \
@@ -279,5 +360,23 @@ h2h_status launch_query(h2h_ctx* ctx, int kind, int exec, const int32_t* dx, con
   return H2H_OK;
 }
 
+
+h2h_status launch_path_walk(h2h_ctx* ctx, int kind, const int64_t* adj_off, const int32_t* adj_nbr,
+                            const int32_t* adj_w, const int32_t* ds, const int32_t* dt, const int32_t* d0,
+                            const int64_t* off, int32_t* verts, int32_t* len, int64_t npairs, hipStream_t s) {
+  if (npairs <= 0) return H2H_OK;
+  QArgs q;
+  q.tin = ctx->d_tin.as<int32_t>();
+  q.lca = ctx->d_lca.as<int64_t>();
+  q.dis_hash = ctx->d_dis_hash.as<int64_t>();
+  q.pos_hash = ctx->d_pos_hash.as<int64_t>();
+  q.pos_pos = ctx->d_pos_pos.as<int32_t>();
+  q.dis = ctx->d_dis.as<int32_t>();
+  q.n = ctx->n;
+  WalkArgs w{adj_off, adj_nbr, adj_w, ds, dt, d0, off, verts, len, npairs, kind};
+  hipLaunchKernelGGL(path_walk_kernel, dim3((unsigned)((npairs + 3) / 4)), dim3(256), 0, s, q, w);
+  H2H_HIP(ctx, hipGetLastError());
+  return H2H_OK;
+}
 
 }  // namespace h2h
