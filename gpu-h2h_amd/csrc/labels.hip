@@ -1506,148 +1506,212 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
 
   host_lap("  plan: chain units, uploads");
   // tasks: bound units, closure lists and apply units (outer columns), BFS
-  // levels (closure and inner columns)
+  // levels (closure and inner columns).  Host threads take contiguous task
+  // ranges into their own arrays (task members are disjoint, so the shared
+  // per-vertex scratch `local` and `thr` see no conflict); the ranges are
+  // then joined in task order with their indices shifted.
   {
     std::vector<std::vector<int32_t>> members(ntasks);  // BFS order inside each task
     for (int i = 0; i < n; i++)
       if (task_of[ctx->bfs[i]] >= 0) members[task_of[ctx->bfs[i]]].push_back(ctx->bfs[i]);
-    std::vector<int32_t> lvl_base(1, 0), starts, lvl_depth, bnodes, banc, tbase, troot, bn_ie(1, 0), ie_w;
-    std::vector<int64_t> tanc;
-    std::vector<uint16_t> tanl;
-    std::vector<uint8_t> ie_s;
-    std::vector<int4> aunits, tinfo;
-    std::vector<int32_t> cl_dst;  // per (node, task depth): index in the apply's closure entries, or -1
-    std::vector<int64_t> cl_rc;   // the apply's closure entries: rows
     const int ra = (int)env_int("H2H_TASK_RA", cfg.task_ra, 8, 16) == 16 ? 16 : 8;
     P.task_ra = ra;
     std::vector<int32_t> local(n, -1);
-    {
-      // reserve the per-node outputs (task depth sums)
-      int64_t nodes = 0, dsum = 0;
-      for (int t = 0; t < ntasks; t++) {
-        const int r = ctx->depth[members[t][0]];
-        nodes += (int64_t)members[t].size();
-        for (int v : members[t]) dsum += ctx->depth[v] - r;
-      }
-      bnodes.reserve(nodes);
-      banc.reserve(nodes);
-      tanc.reserve(dsum);
-      tanl.reserve(dsum);
-      cl_dst.reserve(dsum + nodes);
-      cl_rc.reserve(dsum + nodes);
-      aunits.reserve(nodes * 2);
-      bunits.reserve(bunits.size() + nodes);
-    }
-    std::vector<BoundUnit> tb_units;
-    std::vector<int4> ta_units;
-    std::vector<uint8_t> has_outer;
-    std::vector<int> anc_local, anc_v;
-    P.tbound_b = (int)bunits.size();
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
     // while they are still in the Infinity Cache
     const int nchunks = (int)env_int("H2H_TASK_CHUNKS", std::min(std::max(cfg.task_chunks, 1), 64), 1, 64);
-    P.tchunk_b.clear();
-    P.tchunk_a.clear();
-    for (int t = 0; t < ntasks; t++) {
-      if ((int64_t)t * nchunks / std::max(ntasks, 1) != (int64_t)(t - 1) * nchunks / std::max(ntasks, 1) || t == 0) {
-        P.tchunk_b.push_back((int)bunits.size());
-        P.tchunk_a.push_back((int)aunits.size());
-      }
-      const auto& mem = members[t];
-      const int rt = mem[0];
-      const int r = ctx->depth[rt];
-      const Chain& fc = ch[cpos[chain_of[ctx->father[rt]]]];
-      tbase.push_back((int32_t)bnodes.size());
-      troot.push_back(r);
-      tinfo.push_back(make_int4(r, fc.path, fc.first, fc.h0));
-      for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
-      tb_units.clear();
-      ta_units.clear();
-      // members with entries above the task: only their bounds are finite,
-      // so only they get bound units and closure entries
-      has_outer.assign(mem.size(), 0);
-      for (size_t k = 0; k < mem.size(); k++) {
-        const int v = mem[k];
-        const int d = ctx->depth[v] - r;
-        const int kg = (int)bnodes.size();  // BFS node index
-        thr[v] = r - 1;
-        int nouter = 0, nouter4 = 0;
-        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1 && ctx->pos_pos[e] < r - 1; e++) {
-          nouter++;
-          nouter4 += prune && ctx->pos_dom[e] ? 0 : 1;
+    struct Part {  // one thread's tasks; node, entry and unit indices local to the part
+      std::vector<int32_t> lvl_base, starts, lvl_depth, bnodes, banc, tbase, troot, bn_ie, ie_w, cl_dst;
+      std::vector<int64_t> tanc, cl_rc;
+      std::vector<uint16_t> tanl;
+      std::vector<uint8_t> ie_s;
+      std::vector<int4> aunits, tinfo;
+      std::vector<BoundUnit> bunits;
+      std::vector<int32_t> chunk_b, chunk_a;  // unit counts where a chunk starts (one per chunk start)
+    };
+    auto run_tasks = [&](int t0, int t1, Part& o) {
+      std::vector<BoundUnit> tb_units;
+      std::vector<int4> ta_units;
+      std::vector<uint8_t> has_outer;
+      std::vector<int> anc_local, anc_v;
+      for (int t = t0; t < t1; t++) {
+        if (t == 0 || (int64_t)t * nchunks / std::max(ntasks, 1) != (int64_t)(t - 1) * nchunks / std::max(ntasks, 1)) {
+          o.chunk_b.push_back((int32_t)o.bunits.size());
+          o.chunk_a.push_back((int32_t)o.aunits.size());
         }
-        has_outer[k] = nouter > 0;
-        // closure entries of v: its in-task ancestors (task depth c < d) and
-        // itself (c = d) that have entries above the task
-        const int sel0 = (int)cl_rc.size();
-        {
+        const auto& mem = members[t];
+        const int rt = mem[0];
+        const int r = ctx->depth[rt];
+        const Chain& fc = ch[cpos[chain_of[ctx->father[rt]]]];
+        o.tbase.push_back((int32_t)o.bnodes.size());
+        o.troot.push_back(r);
+        o.tinfo.push_back(make_int4(r, fc.path, fc.first, fc.h0));
+        for (size_t k = 0; k < mem.size(); k++) local[mem[k]] = (int32_t)k;
+        tb_units.clear();
+        ta_units.clear();
+        // members with entries above the task: only their bounds are finite,
+        // so only they get bound units and closure entries
+        has_outer.assign(mem.size(), 0);
+        for (size_t k = 0; k < mem.size(); k++) {
+          const int v = mem[k];
+          const int d = ctx->depth[v] - r;
+          const int kg = (int)o.bnodes.size();  // BFS node index (local to the part)
+          thr[v] = r - 1;
+          int nouter = 0, nouter4 = 0;
+          for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1 && ctx->pos_pos[e] < r - 1; e++) {
+            nouter++;
+            nouter4 += prune && ctx->pos_dom[e] ? 0 : 1;
+          }
+          has_outer[k] = nouter > 0;
+          // closure entries of v: its in-task ancestors (task depth c < d) and
+          // itself (c = d) that have entries above the task
+          const int sel0 = (int)o.cl_rc.size();
+          {
+            int x = v;
+            anc_local.resize(d + 1);
+            anc_v.resize(d + 1);
+            anc_local[d] = (int)k;
+            anc_v[d] = v;
+            for (int q = d - 1; q >= 0; q--) {
+              x = ctx->father[x];
+              anc_local[q] = local[x];
+              anc_v[q] = x;
+            }
+            for (int c = 0; c <= d; c++) {
+              const bool keep = has_outer[anc_local[c]];
+              o.cl_dst.push_back(keep ? (int32_t)o.cl_rc.size() : -1);  // at banc[kg] + kg + c
+              if (keep) o.cl_rc.push_back(ctx->dis_hash[anc_v[c]]);
+            }
+          }
+          const int nsel = (int)o.cl_rc.size() - sel0;
+          for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
+            BoundUnit b;
+            b.v = v;
+            b.c0 = q0;
+            b.L = r - 1;
+            b.nouter = nouter;
+            b.path = fc.path;
+            b.cfirst = fc.first;
+            b.ch0 = fc.h0;
+            b.nouter4 = nouter4;
+            if (nouter > 0) tb_units.push_back(b);
+          }
+          for (int q0 = 0; q0 < r - 1; q0 += 64 * ra) ta_units.push_back(make_int4(kg, q0, (r - 1) | (nsel << 25), sel0));
+          if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
+            o.starts.push_back(kg);
+            o.lvl_depth.push_back(ctx->depth[v]);
+          }
+          o.bnodes.push_back(v);
+          o.banc.push_back((int32_t)o.tanc.size());
+          const size_t at = o.tanc.size();
+          o.tanc.resize(at + d);
+          o.tanl.resize(at + d);
           int x = v;
-          anc_local.resize(d + 1);
-          anc_v.resize(d + 1);
-          anc_local[d] = (int)k;
-          anc_v[d] = v;
           for (int q = d - 1; q >= 0; q--) {
             x = ctx->father[x];
-            anc_local[q] = local[x];
-            anc_v[q] = x;
+            o.tanc[at + q] = ctx->dis_hash[x];
+            o.tanl[at + q] = (uint16_t)local[x];
           }
-          for (int c = 0; c <= d; c++) {
-            const bool keep = has_outer[anc_local[c]];
-            cl_dst.push_back(keep ? (int32_t)cl_rc.size() : -1);  // at banc[kg] + kg + c
-            if (keep) cl_rc.push_back(ctx->dis_hash[anc_v[c]]);
+          for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++) {
+            const int pp = ctx->pos_pos[e];
+            if (pp >= r - 1) {
+              o.ie_s.push_back((uint8_t)(pp - (r - 1)));
+              o.ie_w.push_back(ctx->pos_w[e]);
+            }
           }
+          o.bn_ie.push_back((int32_t)o.ie_w.size());
         }
-        const int nsel = (int)cl_rc.size() - sel0;
+        o.lvl_base.push_back((int32_t)o.lvl_depth.size());
+        // the task's units pass by pass: neighbouring units (one workgroup's
+        // wavefronts) are sibling nodes on the same columns, sharing the
+        // ancestor rows they read through the CU's L1
         for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
-          BoundUnit b;
-          b.v = v;
-          b.c0 = q0;
-          b.L = r - 1;
-          b.nouter = nouter;
-          b.path = fc.path;
-          b.cfirst = fc.first;
-          b.ch0 = fc.h0;
-          b.nouter4 = nouter4;
-          if (nouter > 0) tb_units.push_back(b);
+          for (const auto& bu : tb_units)
+            if (bu.c0 == q0) o.bunits.push_back(bu);
+          for (const auto& xu : ta_units)
+            if (xu.y == q0) o.aunits.push_back(xu);  // (apply units of 64 ra columns start on these too)
         }
-        for (int q0 = 0; q0 < r - 1; q0 += 64 * ra) ta_units.push_back(make_int4(kg, q0, (r - 1) | (nsel << 25), sel0));
-        if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
-          starts.push_back(kg);
-          lvl_depth.push_back(ctx->depth[v]);
-        }
-        bnodes.push_back(v);
-        banc.push_back((int32_t)tanc.size());
-        const size_t at = tanc.size();
-        tanc.resize(at + d);
-        tanl.resize(at + d);
-        int x = v;
-        for (int q = d - 1; q >= 0; q--) {
-          x = ctx->father[x];
-          tanc[at + q] = ctx->dis_hash[x];
-          tanl[at + q] = (uint16_t)local[x];
-        }
-
-        for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++) {
-          const int pp = ctx->pos_pos[e];
-          if (pp >= r - 1) {
-            ie_s.push_back((uint8_t)(pp - (r - 1)));
-            ie_w.push_back(ctx->pos_w[e]);
-          }
-        }
-        bn_ie.push_back((int32_t)ie_w.size());
+        for (int v : mem) local[v] = -1;
       }
-      lvl_base.push_back((int32_t)lvl_depth.size());
-      // the task's units pass by pass: neighbouring units (one workgroup's
-      // wavefronts) are sibling nodes on the same columns, sharing the
-      // ancestor rows they read through the CU's L1
-      for (int q0 = 0; q0 < r - 1; q0 += 64 * R) {
-        for (const auto& b : tb_units)
-          if (b.c0 == q0) bunits.push_back(b);
-        for (const auto& x : ta_units)
-          if (x.y == q0) aunits.push_back(x);  // (apply units of 64 ra columns start on these too)
+    };
+    // contiguous task ranges of about equal node counts
+    const int T = std::max(1, std::min({16, (int)std::thread::hardware_concurrency(), std::max(ntasks, 1)}));
+    std::vector<int> cut(T + 1, ntasks);
+    {
+      int64_t tot = 0, acc = 0;
+      for (int t = 0; t < ntasks; t++) tot += (int64_t)members[t].size();
+      cut[0] = 0;
+      int p = 1;
+      for (int t = 0; t < ntasks && p < T; t++) {
+        acc += (int64_t)members[t].size();
+        while (p < T && acc * T >= tot * p) cut[p++] = t + 1;
       }
-      for (int v : mem) local[v] = -1;
     }
+    std::vector<Part> parts(T);
+    auto on_threads = [&](auto&& fn) {  // fn(part index) on T threads
+      std::vector<std::thread> pool;
+      for (int i = 1; i < T; i++) pool.emplace_back(fn, i);
+      fn(0);
+      for (auto& th : pool) th.join();
+    };
+    on_threads([&](int i) { run_tasks(cut[i], cut[i + 1], parts[i]); });
+    // join the parts in task order, shifting their local indices: each
+    // part's offsets in the joined arrays, then the copies on the threads
+    struct Base {
+      size_t lv, st, nb, tk, ie, cd, an, cl, au, bu;
+    };
+    std::vector<Base> base(T + 1);
+    base[0] = Base{0, 0, 0, 0, 0, 0, 0, 0, 0, bunits.size()};
+    for (int i = 0; i < T; i++) {
+      const Part& o = parts[i];
+      const Base& b = base[i];
+      base[i + 1] = Base{b.lv + o.lvl_depth.size(), b.st + o.starts.size(), b.nb + o.bnodes.size(),
+                         b.tk + o.tbase.size(), b.ie + o.ie_w.size(), b.cd + o.cl_dst.size(),
+                         b.an + o.tanc.size(), b.cl + o.cl_rc.size(), b.au + o.aunits.size(),
+                         b.bu + o.bunits.size()};
+    }
+    const Base& end = base[T];
+    std::vector<int32_t> lvl_base(end.tk + 1, 0), starts(end.st), lvl_depth(end.lv), bnodes(end.nb), banc(end.nb),
+        tbase(end.tk), troot(end.tk), bn_ie(end.nb + 1, 0), ie_w(end.ie), cl_dst(end.cd);
+    std::vector<int64_t> tanc(end.an), cl_rc(end.cl);
+    std::vector<uint16_t> tanl(end.an);
+    std::vector<uint8_t> ie_s(end.ie);
+    std::vector<int4> aunits(end.au), tinfo(end.tk);
+    P.tbound_b = (int)bunits.size();
+    bunits.resize(end.bu);
+    on_threads([&](int i) {
+      const Part& o = parts[i];
+      const Base& b = base[i];
+      const int32_t lv0 = (int32_t)b.lv, nb0 = (int32_t)b.nb, ie0 = (int32_t)b.ie, an0 = (int32_t)b.an,
+                    cl0 = (int32_t)b.cl;
+      for (size_t k = 0; k < o.lvl_base.size(); k++) lvl_base[b.tk + 1 + k] = o.lvl_base[k] + lv0;
+      for (size_t k = 0; k < o.starts.size(); k++) starts[b.st + k] = o.starts[k] + nb0;
+      std::copy(o.lvl_depth.begin(), o.lvl_depth.end(), lvl_depth.begin() + b.lv);
+      std::copy(o.bnodes.begin(), o.bnodes.end(), bnodes.begin() + b.nb);
+      for (size_t k = 0; k < o.banc.size(); k++) banc[b.nb + k] = o.banc[k] + an0;
+      for (size_t k = 0; k < o.tbase.size(); k++) tbase[b.tk + k] = o.tbase[k] + nb0;
+      std::copy(o.troot.begin(), o.troot.end(), troot.begin() + b.tk);
+      for (size_t k = 0; k < o.bn_ie.size(); k++) bn_ie[b.nb + 1 + k] = o.bn_ie[k] + ie0;
+      std::copy(o.ie_w.begin(), o.ie_w.end(), ie_w.begin() + b.ie);
+      std::copy(o.ie_s.begin(), o.ie_s.end(), ie_s.begin() + b.ie);
+      for (size_t k = 0; k < o.cl_dst.size(); k++) cl_dst[b.cd + k] = o.cl_dst[k] < 0 ? -1 : o.cl_dst[k] + cl0;
+      std::copy(o.tanc.begin(), o.tanc.end(), tanc.begin() + b.an);
+      std::copy(o.tanl.begin(), o.tanl.end(), tanl.begin() + b.an);
+      std::copy(o.cl_rc.begin(), o.cl_rc.end(), cl_rc.begin() + b.cl);
+      std::copy(o.tinfo.begin(), o.tinfo.end(), tinfo.begin() + b.tk);
+      for (size_t k = 0; k < o.aunits.size(); k++) {
+        const int4 u = o.aunits[k];
+        aunits[b.au + k] = make_int4(u.x + nb0, u.y, u.z, u.w + cl0);
+      }
+      std::copy(o.bunits.begin(), o.bunits.end(), bunits.begin() + b.bu);
+    });
+    P.tchunk_b.clear();
+    P.tchunk_a.clear();
+    for (int i = 0; i < T; i++)
+      for (size_t c = 0; c < parts[i].chunk_b.size(); c++) {
+        P.tchunk_b.push_back(parts[i].chunk_b[c] + (int)base[i].bu);
+        P.tchunk_a.push_back(parts[i].chunk_a[c] + (int)base[i].au);
+      }
     host_lap("  plan: task lists");
     P.tbound_n = (int)bunits.size() - P.tbound_b;
     P.tchunk_b.push_back((int)bunits.size());
