@@ -593,26 +593,36 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
   ctx->t_contract_us = ms * 1000.0;
   if (usec) *usec = ctx->t_contract_us;
   host_lap("  contraction: bags downloaded");
-  // dominated entries come back as -(w + 1)
+  // dominated entries come back as -(w + 1); each bag is sorted by
+  // neighbour id (the device writes them in column order, which is
+  // ascending id inside a partition already) -- vertex ranges on host threads
   ctx->bag_dom.assign(nb, 0);
-  for (int64_t e = 0; e < nb; e++)
-    if (ctx->bag_w[e] < 0) {
-      ctx->bag_dom[e] = 1;
-      ctx->bag_w[e] = -ctx->bag_w[e] - 1;
+  auto decode = [&](int v0, int v1) {
+    std::vector<std::tuple<int32_t, int32_t, uint8_t>> t;
+    for (int v = v0; v < v1; v++) {
+      const int64_t b0 = ctx->bag_off[v], b1 = ctx->bag_off[v + 1];
+      bool sorted = true;
+      for (int64_t e = b0; e < b1; e++) {
+        if (ctx->bag_w[e] < 0) {
+          ctx->bag_dom[e] = 1;
+          ctx->bag_w[e] = -ctx->bag_w[e] - 1;
+        }
+        if (e > b0 && ctx->bag_nbr[e - 1] > ctx->bag_nbr[e]) sorted = false;
+      }
+      if (!sorted) {
+        t.clear();
+        for (int64_t e = b0; e < b1; e++) t.emplace_back(ctx->bag_nbr[e], ctx->bag_w[e], ctx->bag_dom[e]);
+        std::sort(t.begin(), t.end());
+        for (int64_t e = b0; e < b1; e++) std::tie(ctx->bag_nbr[e], ctx->bag_w[e], ctx->bag_dom[e]) = t[e - b0];
+      }
     }
-  // sort each bag by neighbour id (the device writes them in column order,
-  // which is ascending id inside a partition already)
-  for (int v = 0; v < n; v++) {
-    int64_t b0 = ctx->bag_off[v], b1 = ctx->bag_off[v + 1];
-    bool sorted = true;
-    for (int64_t e = b0 + 1; e < b1; e++)
-      if (ctx->bag_nbr[e - 1] > ctx->bag_nbr[e]) { sorted = false; break; }
-    if (!sorted) {
-      std::vector<std::tuple<int32_t, int32_t, uint8_t>> t;
-      for (int64_t e = b0; e < b1; e++) t.emplace_back(ctx->bag_nbr[e], ctx->bag_w[e], ctx->bag_dom[e]);
-      std::sort(t.begin(), t.end());
-      for (int64_t e = b0; e < b1; e++) std::tie(ctx->bag_nbr[e], ctx->bag_w[e], ctx->bag_dom[e]) = t[e - b0];
-    }
+  };
+  {
+    const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (int i = 1; i < T; i++) pool.emplace_back(decode, (int)((int64_t)n * i / T), (int)((int64_t)n * (i + 1) / T));
+    decode(0, (int)((int64_t)n / T));
+    for (auto& th : pool) th.join();
   }
   return H2H_OK;
 }

@@ -14,8 +14,8 @@
 //   then a 64-bit FNV-1a-style hash of the byte stream after the header,
 //   taken 8 bytes at a time (a final partial word byte by byte).
 // Loading rebuilds the tree layout from the bags (father = deepest bag
-// member, as for h2h_labels_upload) and scatters the rows into it, so the
-// file does not depend on how the saving process laid its arrays out.  A
+// member, as for h2h_labels_upload) and places the rows in it by vertex, so
+// the file does not depend on how the saving process laid its arrays out.  A
 // file without labels (a few percent of the size) has them rebuilt on the
 // device by the batched build, which takes milliseconds where reading the
 // rows takes a large part of a second per GB.
@@ -221,15 +221,19 @@ h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
     }
   if ((st = upload_index(ctx))) return st;
   if (!h.has_labels) return run_label_build(ctx, h.label_kind, H2H_METHOD_BATCHED, nullptr);
-  // the saved rows, scattered into this layout
-  std::vector<int32_t> dis(h.label_entries);
-  int64_t k = 0;
-  for (int v = 0; v < n; v++) {
-    std::copy(rows.begin() + k, rows.begin() + k + depth[v], dis.begin() + ctx->dis_hash[v]);
-    k += depth[v];
+  // the saved rows in this layout: the layout keeps rows in vertex order
+  // (as the file does), so they go up as read; a layout in another order
+  // gets them scattered
+  bool same = true;
+  for (int64_t v = 0, k = 0; v < n && same; k += depth[v], v++) same = ctx->dis_hash[v] == k;
+  std::vector<int32_t> scattered;
+  if (!same) {
+    scattered.resize(h.label_entries);
+    for (int64_t v = 0, k = 0; v < n; k += depth[v], v++)
+      std::copy(rows.begin() + k, rows.begin() + k + depth[v], scattered.begin() + ctx->dis_hash[v]);
   }
-  H2H_HIP(ctx, hipMemcpyAsync(ctx->d_dis.p, dis.data(), h.label_entries * sizeof(int32_t), hipMemcpyHostToDevice,
-                              ctx->stream));
+  H2H_HIP(ctx, hipMemcpyAsync(ctx->d_dis.p, same ? rows.data() : scattered.data(), h.label_entries * sizeof(int32_t),
+                              hipMemcpyHostToDevice, ctx->stream));
   H2H_HIP(ctx, hipStreamSynchronize(ctx->stream));
   ctx->label_kind = h.label_kind;
   ctx->labels_ready = true;
