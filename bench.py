@@ -319,6 +319,8 @@ def main():
     ap.add_argument("--no-scale", action="store_true", help="skip the FLA-scale (1.07M vertices) section")
     ap.add_argument("--e-scale", action="store_true",
                     help="add the E-scale section (3.6M vertices, 1e7 rank-mixed queries; minutes more)")
+    ap.add_argument("--only-queries", action="store_true",
+                    help="contract, build once, run the 1e6-pair query batches and the calibration copy (counter runs)")
     ap.add_argument("--only-build", action="store_true",
                     help="label build steps and the profiled pass only (counter runs)")
     args = ap.parse_args()
@@ -342,6 +344,21 @@ def main():
     t0 = time.perf_counter()
     p.build_labels(h2h.L4, h2h.BATCHED)          # first build: plan, graph capture, one replay
     plan_build_s = time.perf_counter() - t0
+    if args.only_queries:
+        # the 1e6-pair query batches alone (for counter runs): each kind and
+        # execution mode three times, then the calibration copy
+        rng = np.random.default_rng(12345)
+        x = rng.integers(0, g.n, 1000000).astype(np.int32)
+        y = rng.integers(0, g.n, 1000000).astype(np.int32)
+        res = {}
+        for kind, kname in ((h2h.QUERY_LCA, "lca"), (h2h.QUERY_NOLCA, "nolca")):
+            for ex, ename in ((h2h.EXEC_PLAIN, "plain"), (h2h.EXEC_BATCHED, "batched")):
+                res[f"{kname}_{ename}_qps"] = 1e6 / (min(p.query(x, y, kind, ex)[1] for _ in range(3)) * 1e-6)
+        bq_lca, bq_nolca = query_bytes(p.tree(), p.labels(with_dis=False), x, y)
+        calibration_copy()
+        print(json.dumps({"workload": args.workload, "queries": res, "bytes_per_query_lca": bq_lca,
+                          "bytes_per_query_nolca": bq_nolca}))
+        return
     tuned, tune_tried = p.tune_batched(h2h.L4, replays=5)
     p.build_labels(h2h.L4, h2h.BATCHED)          # plans + graph capture
     for _ in range(args.warmup):
