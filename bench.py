@@ -212,8 +212,8 @@ def scale_run(cores, steps):
         res["contraction_device_s"] = p.contract(1, h2h.CONTRACT_GPU_ORDER, cores) * 1e-6
         entries = p.info()["label_entries"]
         tuned, tried = p.tune_batched(h2h.L4, replays=3)
-        res["tuned"] = {k: tuned[k] for k in ("task_nodes", "chain_nodes", "task_ra", "task_chunks", "tri_product",
-                                               "chain_waves", "task_waves")}
+        res["tuned"] = {k: tuned[k] for k in ("task_form", "task_nodes", "chain_nodes", "task_ra", "task_chunks",
+                                               "tri_product", "chain_waves", "task_waves")}
         res["tune_candidates"] = len(tried)
         p.build_labels(h2h.L4, h2h.BATCHED)
         t0 = time.perf_counter()
@@ -520,8 +520,8 @@ def main():
                                f"{2 * g.m / g.n:.2f}, coordinate bisection H={H}, ChangeToGlobalHeight=1, "
                                "GPU-order contraction, L4 labels, batched build",
                    "label_entries": entries, "tree_height": info["tree_height"], "levels": info["levels"],
-                   "tuned": {k: tuned[k] for k in ("task_nodes", "chain_nodes", "task_ra", "task_chunks",
-                                                   "tri_product", "chain_waves", "task_waves")},
+                   "tuned": {k: tuned[k] for k in ("task_form", "task_nodes", "chain_nodes", "task_ra",
+                                                   "task_chunks", "tri_product", "chain_waves", "task_waves")},
                    "tune_candidates": len(tune_tried)},
         "label_build_s": ms_per_step / 1000.0,
         "label_build_device_s": float(np.median(dev)) * 1e-6,

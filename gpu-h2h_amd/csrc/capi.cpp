@@ -326,6 +326,7 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
       o.tri_product = c.tri_product;
       o.chain_waves = c.chain_waves;
       o.task_waves = c.task_waves;
+      o.task_form = c.task_form;
       o.build_us = us;
     }
     nt++;
@@ -354,6 +355,7 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
     }
     return H2H_OK;
   };
+  if ((st = try_field(&BatchedConfig::task_form, {0}))) return st;
   if ((st = try_field(&BatchedConfig::task_nodes, {64, 128, 256}))) return st;
   if ((st = try_field(&BatchedConfig::chain_nodes, {64}))) return st;
   if ((st = try_field(&BatchedConfig::task_ra, {16}))) return st;
@@ -373,6 +375,7 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
     best->tri_product = cur.tri_product;
     best->chain_waves = cur.chain_waves;
     best->task_waves = cur.task_waves;
+    best->task_form = cur.task_form;
     best->build_us = cur_us;
   }
   if (ntried) *ntried = nt;
@@ -390,6 +393,7 @@ h2h_status h2h_get_batched_config(h2h_ctx* ctx, h2h_tune_result* cfg) {
   cfg->tri_product = c.tri_product;
   cfg->chain_waves = c.chain_waves;
   cfg->task_waves = c.task_waves;
+  cfg->task_form = c.task_form;
   cfg->build_us = 0;
   return H2H_OK;
 }
@@ -401,10 +405,10 @@ h2h_status h2h_set_batched_config(h2h_ctx* ctx, const h2h_tune_result* cfg) {
   auto in = [](int x, int lo, int hi) { return x >= lo && x <= hi; };
   if (!in(cfg->task_nodes, 0, 256) || !in(cfg->chain_nodes, 1, 128) || (cfg->task_ra != 8 && cfg->task_ra != 16) ||
       !in(cfg->task_chunks, 1, 64) || !in(cfg->tri_product, 0, 1) || (cfg->chain_waves != 1 && cfg->chain_waves != 4) ||
-      (cfg->task_waves != 1 && cfg->task_waves != 4))
+      (cfg->task_waves != 1 && cfg->task_waves != 4) || !in(cfg->task_form, 0, 1))
     return set_err(ctx, H2H_ERR_ARG,
                    "batched config: task_nodes 0..256, chain_nodes 1..128, task_ra 8 or 16, task_chunks 1..64, "
-                   "tri_product 0 or 1, chain_waves and task_waves 1 or 4");
+                   "tri_product 0 or 1, chain_waves and task_waves 1 or 4, task_form 0 or 1");
   BatchedConfig c;
   c.task_nodes = cfg->task_nodes;
   c.chain_nodes = cfg->chain_nodes;
@@ -413,6 +417,7 @@ h2h_status h2h_set_batched_config(h2h_ctx* ctx, const h2h_tune_result* cfg) {
   c.tri_product = cfg->tri_product;
   c.chain_waves = cfg->chain_waves;
   c.task_waves = cfg->task_waves;
+  c.task_form = cfg->task_form;
   ctx->bcfg = c;
   ctx->lplan[H2H_METHOD_BATCHED].reset();
   return H2H_OK;

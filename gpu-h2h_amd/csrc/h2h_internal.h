@@ -90,6 +90,8 @@ struct LabelPlan {
   bool task_levels = false;  // tasks by levels (one launch) or by closure (bound, close, apply, inner)
   int task_nunits = 0, tbound_b = 0, tbound_n = 0;
   std::vector<int> tchunk_b, tchunk_a;  // task chunks: bound and apply unit offsets
+  bool task_lform = false;              // outer columns level by level (entries: self bound, in-task bag)
+  std::vector<int> tlev_a;              // level form: apply unit offsets per task depth
   int ntasks = 0;
   int64_t task_nodes_n = 0;
   // BATCHED top part: the tree above the subtree tasks as vertical chains
@@ -119,6 +121,8 @@ struct LabelPlan {
     task_nunits = tbound_b = tbound_n = 0;
     task_nodes_n = 0;
     pruned = false;
+    task_lform = false;
+    tlev_a.clear();
     rounds.clear();
     nt = tbo = tro = pro = dis_entries = 0;
     for (int k = 0; k < 3; k++) kcl_b[k] = kcl_n[k] = 0;
@@ -139,6 +143,7 @@ struct BatchedConfig {
   int task_chunks = 1;
   int tri_product = 1;
   int chain_waves = 4;  // wavefronts per chain node's bound unit (1 or 4)
+  int task_form = 1;    // task rows: 1 level form (one launch per task depth), 0 closure form
   int task_waves = 1;   // wavefronts per task node's bound unit (1 or 4)
 };
 

@@ -1016,6 +1016,50 @@ __global__ void __launch_bounds__(256) task_close_kernel(TaskArgs t) {
   }
 }
 
+// Inner columns r - 1 .. r - 2 + d of task node k (vertex v at task depth
+// d, task root depth r) from its whole bag, one wavefront, lane = column;
+// then its self column.  The in-task ancestors' rows are final.
+template <bool L4, bool IDX32>
+__device__ inline void task_inner_node(const TaskArgs& t, int k, int r, int d, int lane) {
+  const LabelArgs& a = t.a;
+  const int v = t.bnodes[k];
+  const int64_t rowv = a.dis_hash[v];
+  if (d > 0) {
+    const int j = r - 1 + min(lane, d - 1);
+    const int64_t arow = t.tanc[t.banc[k] + min(lane, d - 1)];  // a_j: in-task ancestor at task depth lane
+    const int64_t e0 = a.pos_hash[v];
+    const int nk = (int)(a.pos_hash[v + 1] - 1 - e0);
+    int32_t x = INF32;
+    for (int kb = 0; kb < nk; kb += 64) {
+      int mp = 0, mw = 0;
+      int64_t mro = 0;
+      if (kb + lane < nk) {
+        mp = a.pos_pos[e0 + kb + lane];
+        mw = a.pos_w[e0 + kb + lane];
+        mro = a.pos_rowoff[e0 + kb + lane];
+      }
+      const int kn = min(64, nk - kb);
+      for (int q0 = 0; q0 < kn; q0 += 4) {
+        int32_t y[4], wv[4];
+        bool use[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int e = min(q0 + q, kn - 1);
+          const int p = __builtin_amdgcn_readlane(mp, e);
+          wv[q] = __builtin_amdgcn_readlane(mw, e);
+          const int64_t ro = readlane64(mro, e);
+          use[q] = (q0 + q < kn) && (L4 || j <= p);
+          y[q] = ld<IDX32>(a.dis, L4 ? ((j <= p) ? (uint64_t)ro + j : (uint64_t)arow + p) : (uint64_t)ro + min(j, p));
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) x = use[q] ? min(x, wv[q] + y[q]) : x;
+      }
+    }
+    if (lane < d) a.dis[rowv + r - 1 + lane] = x;
+  }
+  if (lane == 0) a.dis[rowv + r - 1 + d] = 0;
+}
+
 // Outer columns of task nodes: one wavefront per (node, 512 columns),
 // min over the closure entries (P, row) of w + dis[row][j], two entries per
 // round of loads.
@@ -1026,6 +1070,11 @@ __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   if (u >= t.nunits) return;  // whole wavefronts
   const LabelArgs& a = t.a;
   const int4 un = t.units[t.ubegin + u];
+  if (un.y < 0) {  // level form: the node's inner columns (un.z = r - 1, un.w = task depth)
+    if (a.l4) task_inner_node<true, IDX32>(t, un.x, un.z + 1, un.w, lane);
+    else task_inner_node<false, IDX32>(t, un.x, un.z + 1, un.w, lane);
+    return;
+  }
   const int k = un.x, c0 = un.y, L = un.z & ((1 << 25) - 1), ne = un.z >> 25;
   const int64_t rowv = a.dis_hash[t.bnodes[k]];
   int32_t acc[RA];
@@ -1071,50 +1120,12 @@ __global__ void __launch_bounds__(256) task_inner_kernel(TaskArgs t) {
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int task = xcd_index(t.ntasks);
   if (task >= t.ntasks) return;  // whole workgroup
-  const LabelArgs& a = t.a;
   const int r = t.troot[task];
   const int l1 = t.lvl_base[task + 1];
   for (int li = t.lvl_base[task]; li < l1; li++) {
     const int n0 = t.lvl_off[li], n1 = t.lvl_off[li + 1];
     const int d = t.lvl_depth[li] - r;  // task depth: inner columns r-1 .. r-2+d, self at r-1+d
-    for (int k = n0 + wid; k < n1; k += 4) {
-      const int v = t.bnodes[k];
-      const int64_t rowv = a.dis_hash[v];
-      if (d > 0) {
-        const int j = r - 1 + min(lane, d - 1);
-        const int64_t arow = t.tanc[t.banc[k] + min(lane, d - 1)];  // a_j: in-task ancestor at task depth lane
-        const int64_t e0 = a.pos_hash[v];
-        const int nk = (int)(a.pos_hash[v + 1] - 1 - e0);
-        int32_t x = INF32;
-        for (int kb = 0; kb < nk; kb += 64) {
-          int mp = 0, mw = 0;
-          int64_t mro = 0;
-          if (kb + lane < nk) {
-            mp = a.pos_pos[e0 + kb + lane];
-            mw = a.pos_w[e0 + kb + lane];
-            mro = a.pos_rowoff[e0 + kb + lane];
-          }
-          const int kn = min(64, nk - kb);
-          for (int q0 = 0; q0 < kn; q0 += 4) {
-            int32_t y[4], wv[4];
-            bool use[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int e = min(q0 + q, kn - 1);
-              const int p = __builtin_amdgcn_readlane(mp, e);
-              wv[q] = __builtin_amdgcn_readlane(mw, e);
-              const int64_t ro = readlane64(mro, e);
-              use[q] = (q0 + q < kn) && (L4 || j <= p);
-              y[q] = ld<IDX32>(a.dis, L4 ? ((j <= p) ? (uint64_t)ro + j : (uint64_t)arow + p) : (uint64_t)ro + min(j, p));
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q++) x = use[q] ? min(x, wv[q] + y[q]) : x;
-          }
-        }
-        if (lane < d) a.dis[rowv + r - 1 + lane] = x;
-      }
-      if (lane == 0) a.dis[rowv + r - 1 + d] = 0;
-    }
+    for (int k = n0 + wid; k < n1; k += 4) task_inner_node<L4, IDX32>(t, k, r, d, lane);
     __syncthreads();
   }
 }
@@ -1520,7 +1531,16 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     // tasks run in chunks: a chunk's bounds are read back by the apply pass
     // while they are still in the Infinity Cache
     const int nchunks = (int)env_int("H2H_TASK_CHUNKS", std::min(std::max(cfg.task_chunks, 1), 64), 1, 64);
+    // level form: the outer columns of a task depth's nodes from their own
+    // bound (weight 0, when they have entries above the task) and their
+    // in-task bag members (weight w, rows final at earlier depths), one
+    // launch per task depth; the closure form reads every ancestor with
+    // entries above the task through the closure P instead
+    const bool lform = env_int("H2H_TASK_FORM", cfg.task_form ? 1 : 0, 0, 1) != 0;
+    P.task_lform = lform;
     struct Part {  // one thread's tasks; node, entry and unit indices local to the part
+      std::vector<int32_t> cl_w;                // level form: entry weights
+      std::vector<std::vector<int4>> lunits;    // level form: apply units by task depth
       std::vector<int32_t> lvl_base, starts, lvl_depth, bnodes, banc, tbase, troot, bn_ie, ie_w, cl_dst;
       std::vector<int64_t> tanc, cl_rc;
       std::vector<uint16_t> tanl;
@@ -1566,7 +1586,18 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
           // closure entries of v: its in-task ancestors (task depth c < d) and
           // itself (c = d) that have entries above the task
           const int sel0 = (int)o.cl_rc.size();
-          {
+          if (lform) {
+            if (has_outer[k]) {
+              o.cl_rc.push_back(ctx->dis_hash[v]);
+              o.cl_w.push_back(0);
+            }
+            for (int64_t e = ctx->pos_hash[v]; e < ctx->pos_hash[v + 1] - 1; e++)
+              if (ctx->pos_pos[e] >= r - 1) {
+                o.cl_rc.push_back(ctx->dis_hash[ctx->pos_id[e]]);
+                o.cl_w.push_back(ctx->pos_w[e]);
+              }
+            for (int c2 = 0; c2 <= d; c2++) o.cl_dst.push_back(-1);  // (no closure pass)
+          } else {
             int x = v;
             anc_local.resize(d + 1);
             anc_v.resize(d + 1);
@@ -1596,7 +1627,20 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
             b.nouter4 = nouter4;
             if (nouter > 0) tb_units.push_back(b);
           }
-          for (int q0 = 0; q0 < r - 1; q0 += 64 * ra) ta_units.push_back(make_int4(kg, q0, (r - 1) | (nsel << 25), sel0));
+          for (int q0 = 0; q0 < r - 1; q0 += 64 * ra) {
+            const int4 au = make_int4(kg, q0, (r - 1) | (nsel << 25), sel0);
+            if (lform && nsel == (has_outer[k] ? 1 : 0)) break;  // the row holds its bound already (roots)
+            if (lform) {
+              if ((int)o.lunits.size() <= d) o.lunits.resize(d + 1);
+              o.lunits[d].push_back(au);
+            } else {
+              ta_units.push_back(au);
+            }
+          }
+          if (lform) {  // the node's inner columns, in the same depth's launch
+            if ((int)o.lunits.size() <= d) o.lunits.resize(d + 1);
+            o.lunits[d].push_back(make_int4(kg, -1, r - 1, d));
+          }
           if (k == 0 || ctx->depth[v] != ctx->depth[mem[k - 1]]) {
             o.starts.push_back(kg);
             o.lvl_depth.push_back(ctx->depth[v]);
@@ -1677,6 +1721,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     std::vector<uint16_t> tanl(end.an);
     std::vector<uint8_t> ie_s(end.ie);
     std::vector<int4> aunits(end.au), tinfo(end.tk);
+    std::vector<int32_t> cl_wh(lform ? end.cl : 0);  // level form: entry weights
     P.tbound_b = (int)bunits.size();
     bunits.resize(end.bu);
     on_threads([&](int i) {
@@ -1698,6 +1743,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
       std::copy(o.tanc.begin(), o.tanc.end(), tanc.begin() + b.an);
       std::copy(o.tanl.begin(), o.tanl.end(), tanl.begin() + b.an);
       std::copy(o.cl_rc.begin(), o.cl_rc.end(), cl_rc.begin() + b.cl);
+      if (lform) std::copy(o.cl_w.begin(), o.cl_w.end(), cl_wh.begin() + b.cl);
       std::copy(o.tinfo.begin(), o.tinfo.end(), tinfo.begin() + b.tk);
       for (size_t k = 0; k < o.aunits.size(); k++) {
         const int4 u = o.aunits[k];
@@ -1712,6 +1758,22 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
         P.tchunk_b.push_back(parts[i].chunk_b[c] + (int)base[i].bu);
         P.tchunk_a.push_back(parts[i].chunk_a[c] + (int)base[i].au);
       }
+    P.tlev_a.clear();
+    if (lform) {
+      // apply units by task depth across all parts (a part's units carry
+      // part-local node and entry indices, shifted here)
+      size_t levels = 0;
+      for (const Part& o : parts) levels = std::max(levels, o.lunits.size());
+      aunits.clear();
+      for (size_t dl = 0; dl < levels; dl++) {
+        P.tlev_a.push_back((int)aunits.size());
+        for (int i = 0; i < T; i++)
+          if (dl < parts[i].lunits.size())
+            for (const int4& uu : parts[i].lunits[dl])
+              aunits.push_back(make_int4(uu.x + (int)base[i].nb, uu.y, uu.z, uu.y < 0 ? uu.w : uu.w + (int)base[i].cl));
+      }
+      P.tlev_a.push_back((int)aunits.size());
+    }
     host_lap("  plan: task lists");
     P.tbound_n = (int)bunits.size() - P.tbound_b;
     P.tchunk_b.push_back((int)bunits.size());
@@ -1733,7 +1795,8 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_cl_ro, cl_rc.data(), std::max<size_t>(cl_rc.size(), 1) * sizeof(int64_t)));
     H2H_TRY(upload(ctx, P.task_cl_dst, cl_dst.data(), std::max<size_t>(cl_dst.size(), 1) * sizeof(int32_t)));
-    H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_rc.size(), 1) * sizeof(int32_t)));
+    if (lform && !cl_wh.empty()) H2H_TRY(upload(ctx, P.task_cl_w, cl_wh.data(), cl_wh.size() * sizeof(int32_t)));
+    else H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_rc.size(), 1) * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_units, aunits.data(), aunits.size() * sizeof(int4)));
     P.ntasks = ntasks;
     P.task_nodes_n = (int64_t)bnodes.size();
@@ -1861,7 +1924,7 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
     t.ubegin = 0;
     t.nunits = P.task_nunits;
     t.ntasks = P.ntasks;
-    const bool task_close = P.ntasks > 0 && !P.task_levels;
+    const bool task_close = P.ntasks > 0 && !P.task_levels && !P.task_lform;
     // The two closures depend only on the tree's bag weights: off the timed
     // path (ev == nullptr) they run on a side stream beside the chain rounds
     // and are joined where their results are first read -- the chains'
@@ -1922,6 +1985,18 @@ h2h_status enqueue_plan(h2h_ctx* ctx, const LabelPlan& P, const LabelArgs& a, hi
       if (P.task_levels) {
         H2H_TRY(timed([&] { launch_task_levels(t, P.tpath.as<int32_t>(), s); }));
         return H2H_OK;
+      }
+      if (P.task_lform) {
+        // level form: every task bound, then the outer columns depth by depth
+        const int b0 = P.tchunk_b.front(), b1 = P.tchunk_b.back();
+        if (b1 > b0)
+          H2H_TRY(timed([&] { launch_bound(a, P.bunits.as<BoundUnit>(), P.tpath.as<int32_t>(), b0, b1 - b0, s, task_split); }));
+        for (size_t dl = 0; dl + 1 < P.tlev_a.size(); dl++) {
+          t.ubegin = P.tlev_a[dl];
+          t.nunits = P.tlev_a[dl + 1] - P.tlev_a[dl];
+          if (t.nunits > 0) H2H_TRY(timed([&] { launch_tasks(t, s, 1); }));
+        }
+        return H2H_OK;  // (the depth launches wrote the inner columns too)
       }
       if (!s2) H2H_TRY(timed([&] { launch_tasks(t, s, 0); }));
       for (size_t k = 0; k + 1 < P.tchunk_b.size(); k++) {

@@ -50,7 +50,7 @@ class ServerCfg(ctypes.Structure):
 class TuneResult(ctypes.Structure):
     _fields_ = [("task_nodes", ctypes.c_int), ("chain_nodes", ctypes.c_int), ("task_ra", ctypes.c_int),
                 ("task_chunks", ctypes.c_int), ("tri_product", ctypes.c_int), ("chain_waves", ctypes.c_int),
-                ("task_waves", ctypes.c_int), ("build_us", ctypes.c_double)]
+                ("task_waves", ctypes.c_int), ("task_form", ctypes.c_int), ("build_us", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -130,6 +130,7 @@ typedef struct {
   int tri_product;  /* 1: L4 chain triangles as the min-plus product; 0: Floyd-Warshall */
   int chain_waves;  /* wavefronts sharing one chain node's bound unit: 1 or 4 */
   int task_waves;   /* wavefronts sharing one task node's bound unit: 1 or 4 */
+  int task_form;    /* task rows: 1 depth by depth from own bound and in-task bag; 0 closure form */
   double build_us;  /* the measured build time of this configuration (h2h_tune_batched) */
 } h2h_tune_result;
 
