@@ -319,6 +319,9 @@ def main():
     ap.add_argument("--no-scale", action="store_true", help="skip the FLA-scale (1.07M vertices) section")
     ap.add_argument("--e-scale", action="store_true",
                     help="add the E-scale section (3.6M vertices, 1e7 rank-mixed queries; minutes more)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the library's default batched-build parameters (counter runs: a profiler's timing "
+                         "would steer the autotuner)")
     ap.add_argument("--only-queries", action="store_true",
                     help="contract, build once, run the 1e6-pair query batches and the calibration copy (counter runs)")
     ap.add_argument("--only-build", action="store_true",
@@ -359,7 +362,10 @@ def main():
         print(json.dumps({"workload": args.workload, "queries": res, "bytes_per_query_lca": bq_lca,
                           "bytes_per_query_nolca": bq_nolca}))
         return
-    tuned, tune_tried = p.tune_batched(h2h.L4, replays=5)
+    if args.no_tune:
+        tuned, tune_tried = dict(p.batched_config(), build_us=None), []
+    else:
+        tuned, tune_tried = p.tune_batched(h2h.L4, replays=5)
     p.build_labels(h2h.L4, h2h.BATCHED)          # plans + graph capture
     for _ in range(args.warmup):
         p.build_labels(h2h.L4, h2h.BATCHED)

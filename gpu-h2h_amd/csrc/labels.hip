@@ -1264,7 +1264,7 @@ constexpr int CHAIN_INNER_MAX = 16384;  // bag entries inside one chain
 
 // Tuning knobs of the batched plan (environment, read when a plan is built):
 // H2H_TASK_NODES caps the nodes of a subtree task (1..256, default
-// n / (8 * CUs) within 64..256), H2H_CHAIN_NODES the nodes of a chain (1..128,
+// n / (2 * CUs) within 64..256), H2H_CHAIN_NODES the nodes of a chain (1..128,
 // default 128), H2H_TASK_LEVELS=1 builds the tasks level by level in one
 // launch instead of by closure (bound, close, apply, inner), H2H_TASK_CHUNKS
 // splits the tasks' bound and apply passes into chunks (default 1).
@@ -1322,7 +1322,7 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   const BatchedConfig& cfg = ctx->bcfg;
   const int64_t smax = env_int("H2H_TASK_NODES",
                                cfg.task_nodes > 0 ? std::min<int64_t>(cfg.task_nodes, TASK_NODES)
-                                                  : std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (8LL * ctx->num_cus))),
+                                                  : std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (2LL * ctx->num_cus))),
                                1, TASK_NODES);
   const int chain_max = (int)env_int("H2H_CHAIN_NODES", std::min(std::max(cfg.chain_nodes, 1), CHAIN_MAX), 1, CHAIN_MAX);
   int ntasks = 0;
