@@ -136,9 +136,9 @@ typedef struct {
 
 /* The project's autotuner for the batched build: times the batched build of
  * the resident index (median of `replays` graph replays, label kind
- * `label_kind`) for each candidate configuration -- task size caps, chain
- * caps, task apply tile widths, task chunking, wavefronts per node in the
- * bound pass -- keeps the
+ * `label_kind`) for each candidate configuration -- task form, task size
+ * caps, chain caps, task apply tile widths, task chunking, wavefronts per
+ * node in the bound pass -- keeps the
  * fastest for every later H2H_METHOD_BATCHED build of this context and
  * returns it in *best (and up to max_tried measured candidates in tried, if
  * not NULL; *ntried gets their count).  Every candidate gives identical

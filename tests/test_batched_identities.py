@@ -12,7 +12,9 @@ kernel:
 * dropping a bag entry that a two-hop path through another bag member
   beats strictly leaves every L4 label unchanged;
 * the closure form of a subtree's outer columns, reading only the ancestors
-  that have entries above the subtree.
+  that have entries above the subtree;
+* the level form of a subtree's outer columns: a node's own bound over its
+  entries above the subtree, then its bag members inside the subtree.
 
 CPU only."""
 import numpy as np
@@ -172,3 +174,26 @@ def test_subtree_outer_columns_skip_ancestors_without_outer_entries(seed):
                             best = min(best, dist[u] + bw[e] + g)
                 assert best == d4[dh[v] + j], f"root={root} v={v} j={j}"
         break  # one subtree per graph keeps the test fast
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_subtree_outer_columns_level_form(seed):
+    n, fa, dep, dh, d3, d4, off, nbr, bw = _index(seed)
+    rng = np.random.default_rng(seed)
+    checked = 0
+    for v in rng.choice(n, min(n, 120), replace=False):
+        path = _path_to_root(fa, int(v))  # root first, v last
+        for root in path[1:]:             # every ancestor below the tree root as the subtree root
+            R = dep[root]
+            for j in range(R - 1):
+                b = INF  # the bound: entries above the subtree
+                inner = INF  # bag members inside the subtree (their rows hold the final values)
+                for e in range(off[v], off[v + 1]):
+                    u, w, pu = nbr[e], bw[e], dep[nbr[e]] - 1
+                    if dep[u] < R:
+                        b = min(b, w + (d4[dh[u] + j] if j <= pu else d4[dh[path[j]] + pu]))
+                    else:
+                        inner = min(inner, w + d4[dh[u] + j])
+                assert min(b, inner) == d4[dh[v] + j], f"v={v} root={root} j={j}"
+                checked += 1
+    assert checked > 0
