@@ -356,7 +356,7 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
     return H2H_OK;
   };
   if ((st = try_field(&BatchedConfig::task_form, {0}))) return st;
-  if ((st = try_field(&BatchedConfig::task_nodes, {64, 128, 256}))) return st;
+  if ((st = try_field(&BatchedConfig::task_nodes, {64, 128, 256, 512, 1024}))) return st;
   if ((st = try_field(&BatchedConfig::chain_nodes, {64}))) return st;
   if ((st = try_field(&BatchedConfig::task_ra, {16}))) return st;
   if ((st = try_field(&BatchedConfig::task_chunks, {2, 4}))) return st;
@@ -403,11 +403,11 @@ h2h_status h2h_set_batched_config(h2h_ctx* ctx, const h2h_tune_result* cfg) {
   if (!cfg) return set_err(ctx, H2H_ERR_ARG, "null config");
   if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before changing the batched plan");
   auto in = [](int x, int lo, int hi) { return x >= lo && x <= hi; };
-  if (!in(cfg->task_nodes, 0, 256) || !in(cfg->chain_nodes, 1, 128) || (cfg->task_ra != 8 && cfg->task_ra != 16) ||
+  if (!in(cfg->task_nodes, 0, 4096) || !in(cfg->chain_nodes, 1, 128) || (cfg->task_ra != 8 && cfg->task_ra != 16) ||
       !in(cfg->task_chunks, 1, 64) || !in(cfg->tri_product, 0, 1) || (cfg->chain_waves != 1 && cfg->chain_waves != 4) ||
       (cfg->task_waves != 1 && cfg->task_waves != 4) || !in(cfg->task_form, 0, 1))
     return set_err(ctx, H2H_ERR_ARG,
-                   "batched config: task_nodes 0..256, chain_nodes 1..128, task_ra 8 or 16, task_chunks 1..64, "
+                   "batched config: task_nodes 0..4096 (above 256: level form only), chain_nodes 1..128, task_ra 8 or 16, task_chunks 1..64, "
                    "tri_product 0 or 1, chain_waves and task_waves 1 or 4, task_form 0 or 1");
   BatchedConfig c;
   c.task_nodes = cfg->task_nodes;

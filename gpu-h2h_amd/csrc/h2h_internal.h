@@ -137,7 +137,7 @@ struct Server;
 
 // Batched-build parameters: defaults, or what h2h_tune_batched chose
 struct BatchedConfig {
-  int task_nodes = 0;  // 0: min(256, max(64, n / (2 CUs)))
+  int task_nodes = 0;  // 0: level form min(512, max(64, n / CUs)), closure form min(256, max(64, n / (2 CUs)))
   int chain_nodes = 128;
   int task_ra = 8;
   int task_chunks = 1;

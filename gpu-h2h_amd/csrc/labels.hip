@@ -957,8 +957,10 @@ hipError_t set_chain_lds_limits() {
 //     for another node of the same launch;
 //   inner columns r - 1 .. depth(v) - 2 (ancestors inside the task): one
 //     workgroup per task, level by level (task_inner_kernel).
-constexpr int TASK_H = 32;       // task height cap
-constexpr int TASK_NODES = 256;  // task size cap: the closure's LDS tile
+constexpr int TASK_H = 32;        // task height cap (the closure form's LDS tile; for the
+                                  // level form, taller tasks add depth launches and measured slower)
+constexpr int TASK_NODES = 256;          // task size cap of the closure form: its LDS tile
+constexpr int TASK_NODES_LEVEL = 4096;   // task size cap of the level form (no tile)
 
 struct TaskArgs {
   LabelArgs a;
@@ -1082,7 +1084,7 @@ __global__ void __launch_bounds__(256) task_apply_kernel(TaskArgs t) {
   for (int r = 0; r < RA; r++) acc[r] = INF32;
   int mw = 0;
   int64_t mro = 0;
-  if (lane < ne) {  // at most TASK_H entries
+  if (lane < ne) {  // at most TASK_H entries (closure form), 1 + TASK_H - 1 (level form)
     mw = t.cl_w[un.w + lane];
     mro = t.cl_ro[un.w + lane];
   }
@@ -1263,8 +1265,9 @@ namespace {
 constexpr int CHAIN_INNER_MAX = 16384;  // bag entries inside one chain
 
 // Tuning knobs of the batched plan (environment, read when a plan is built):
-// H2H_TASK_NODES caps the nodes of a subtree task (1..256, default
-// n / (2 * CUs) within 64..256), H2H_CHAIN_NODES the nodes of a chain (1..128,
+// H2H_TASK_NODES caps the nodes of a subtree task (level form 1..4096,
+// default n / CUs within 64..512; closure form 1..256, default n / (2 * CUs)
+// within 64..256), H2H_CHAIN_NODES the nodes of a chain (1..128,
 // default 128), H2H_TASK_LEVELS=1 builds the tasks level by level in one
 // launch instead of by closure (bound, close, apply, inner), H2H_TASK_CHUNKS
 // splits the tasks' bound and apply passes into chunks (default 1).
@@ -1320,10 +1323,14 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
   // maximal subtrees of at most smax nodes and TASK_H levels become tasks
   std::vector<int32_t> task_of(n, -1);
   const BatchedConfig& cfg = ctx->bcfg;
-  const int64_t smax = env_int("H2H_TASK_NODES",
-                               cfg.task_nodes > 0 ? std::min<int64_t>(cfg.task_nodes, TASK_NODES)
-                                                  : std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (2LL * ctx->num_cus))),
-                               1, TASK_NODES);
+  const bool lform = env_int("H2H_TASK_FORM", cfg.task_form ? 1 : 0, 0, 1) != 0;
+  const int64_t tcap = lform ? TASK_NODES_LEVEL : TASK_NODES;
+  const int64_t smax =
+      env_int("H2H_TASK_NODES",
+              cfg.task_nodes > 0 ? std::min<int64_t>(cfg.task_nodes, tcap)
+              : lform            ? std::min<int64_t>(512, std::max<int64_t>(64, n / ctx->num_cus))
+                                 : std::min<int64_t>(TASK_NODES, std::max<int64_t>(64, n / (2LL * ctx->num_cus))),
+              1, tcap);
   const int chain_max = (int)env_int("H2H_CHAIN_NODES", std::min(std::max(cfg.chain_nodes, 1), CHAIN_MAX), 1, CHAIN_MAX);
   int ntasks = 0;
   {
@@ -1536,7 +1543,6 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     // in-task bag members (weight w, rows final at earlier depths), one
     // launch per task depth; the closure form reads every ancestor with
     // entries above the task through the closure P instead
-    const bool lform = env_int("H2H_TASK_FORM", cfg.task_form ? 1 : 0, 0, 1) != 0;
     P.task_lform = lform;
     struct Part {  // one thread's tasks; node, entry and unit indices local to the part
       std::vector<int32_t> cl_w;                // level form: entry weights

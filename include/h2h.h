@@ -123,7 +123,8 @@ h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* us
 /* Batched-build parameters (H2H_METHOD_BATCHED).  0 in a field of
  * h2h_tune_result means "automatic" where noted. */
 typedef struct {
-  int task_nodes;   /* subtree task size cap, 1..256 (0: min(256, max(64, n / (2 CUs)))) */
+  int task_nodes;   /* subtree task size cap: level form 1..4096 (0: min(512, max(64, n / CUs))),
+                       closure form 1..256 (0: min(256, max(64, n / (2 CUs)))) */
   int chain_nodes;  /* chain length cap, 1..128 */
   int task_ra;      /* task apply columns per lane: 8 or 16 */
   int task_chunks;  /* task bound/apply passes in this many chunks, 1..64 */

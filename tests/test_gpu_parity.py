@@ -214,7 +214,8 @@ def test_batched_alternative_paths(monkeypatch, tri, prune, ra):
 @pytest.mark.parametrize("cfg", [dict(chain_waves=1, task_waves=4), dict(task_nodes=64, chain_nodes=64, task_chunks=3),
                                  dict(task_nodes=200, chain_nodes=37, task_ra=16, tri_product=0),
                                  dict(task_form=0), dict(task_form=0, task_chunks=3, task_ra=16),
-                                 dict(task_form=1, task_nodes=1), dict(task_form=1, task_nodes=256, task_waves=4)])
+                                 dict(task_form=1, task_nodes=1), dict(task_form=1, task_nodes=256, task_waves=4),
+                                 dict(task_form=1, task_nodes=4096, task_ra=16)])
 def test_batched_config_variants(cfg):
     """Batched-build parameters set through the C ABI (one wavefront per chain
     node's bound unit and four per task node's; other task and chain caps;

@@ -162,7 +162,7 @@ def test_batched_config_round_trip():
                          task_waves=1, task_form=1)
         p.set_batched_config(task_nodes=96, chain_waves=1, task_waves=4)
         assert p.batched_config() == dict(d, task_nodes=96, chain_waves=1, task_waves=4)
-        for bad in (dict(task_nodes=257), dict(chain_nodes=0), dict(task_ra=12), dict(task_chunks=0),
+        for bad in (dict(task_nodes=4097), dict(chain_nodes=0), dict(task_ra=12), dict(task_chunks=0),
                     dict(tri_product=2), dict(chain_waves=2), dict(task_waves=0), dict(task_form=2)):
             with pytest.raises(h2h.H2HError) as e:
                 p.set_batched_config(**bad)
