@@ -29,7 +29,8 @@
 //            rule of kernel_functions.cu:717-744)
 //
 // BATCHED (one hipGraph, replayed)
-//   Maximal subtrees of at most TASK_NODES nodes and TASK_H levels are tasks;
+//   Maximal subtrees of at most TASK_NODES_LEVEL (level form) or TASK_NODES
+//   (closure form) nodes and TASK_H levels are tasks;
 //   the tree above them is cut into vertical chains of at most 128 nodes (a
 //   chain is a run of nodes each with a single child outside the tasks: the
 //   separator vertices of one partition cell).  Chains of equal chain depth
@@ -47,14 +48,19 @@
 //     bound_kernel        B for every (chain node, 512 columns);
 //     chain_apply_kernel  P (x) B for every (chain, 64 columns);
 //     chain_cross_kernel  t_i(m) for every chain node (L4);
-//     chain_fw_kernel     the triangle by Floyd-Warshall, one workgroup per
-//                         chain.  (The chain nodes' bags hold most of the
-//                         chain above them, so the row recurrence would be
-//                         as much work, run by one wavefront.)
-//   The closures P of all chains are one more launch, ahead of the rounds.
-//   A task's rows split the same way at its root: B from the entries above
-//   the task (bound_kernel), P from task_close_kernel, the outer columns by
-//   task_apply_kernel and the inner columns level by level inside one
+//     chain_tri_kernel    the L4 triangle as one min-plus product of P with
+//                         the cross terms (chain_fw_kernel: Floyd-Warshall,
+//                         for L3 and for the root chain).  (The chain nodes'
+//                         bags hold most of the chain above them, so the row
+//                         recurrence would be as much work, run by one
+//                         wavefront.)
+//   The closures P of all chains run on a side stream, by size class.
+//   A task's rows, level form (the default): B from the entries above the
+//   task (bound_kernel), then one task_apply_kernel launch per task depth
+//   over every task: outer columns from the node's own B and its in-task bag
+//   members' rows, inner columns (one wavefront per node) from its whole bag.
+//   Closure form: P from task_close_kernel, the outer columns by one
+//   task_apply_kernel launch, the inner columns level by level inside one
 //   workgroup per task (task_inner_kernel).
 //   The L4 gathers dis[a_j][p] of chain and task rows read a transposed copy
 //   of the chain rows (tb[p][t] = dis[chain node t][p], written with the
