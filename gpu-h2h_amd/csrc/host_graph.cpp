@@ -76,8 +76,27 @@ h2h_status build_partition_plans(h2h_ctx* ctx, int C) {
   std::vector<int32_t> stamp(n, -1);
   std::vector<int32_t> cap(n, 0);
   for (int v = 0; v < n; v++) cap[v] = (int32_t)(ctx->adj_off[v + 1] - ctx->adj_off[v]);
-  std::vector<int32_t> members, outc;
   ctx->top_base = -1;
+  // the partitions of a level are independent: their present members (and
+  // stamps), then their outside-neighbour counts, on host threads in two
+  // passes (the counts read other partitions' stamps); the record is then
+  // appended in heap order
+  const int T = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  struct PartMembers {
+    std::vector<int32_t> members, outc;
+    int32_t ncand = 0;
+  };
+  std::vector<PartMembers> pm;
+  auto on_threads = [&](int64_t count, auto&& fn) {  // fn(i) for i < count
+    std::vector<std::thread> pool;
+    const int tn = (int)std::min<int64_t>(T, std::max<int64_t>(count, 1));
+    auto body = [&](int k) {
+      for (int64_t i = (int64_t)count * k / tn; i < (int64_t)count * (k + 1) / tn; i++) fn(i);
+    };
+    for (int k = 1; k < tn; k++) pool.emplace_back(body, k);
+    body(0);
+    for (auto& th : pool) th.join();
+  };
 
   for (int h = H; h >= 1; h--) {
     const int64_t lo = (1LL << (h - 1)) - 1, hi = (1LL << h) - 2;
@@ -86,27 +105,37 @@ h2h_status build_partition_plans(h2h_ctx* ctx, int C) {
     plan.height = h;
     plan.memb_off.push_back(0);
     if (h == C) ctx->top_base = (int32_t)ctx->elim.size();
-    for (int64_t p = hi; p >= lo; p--) {
-      int32_t l = ctx->cut_lr[2 * p], r = ctx->cut_lr[2 * p + 1];
-      members.clear();
+    pm.assign(hi - lo + 1, PartMembers());
+    on_threads(hi - lo + 1, [&](int64_t i) {
+      const int64_t p = hi - i;
+      PartMembers& m = pm[i];
+      const int32_t l = ctx->cut_lr[2 * p], r = ctx->cut_lr[2 * p + 1];
       for (int32_t j = l; j < r; j++) {
-        int32_t v = ctx->index_ids[j];
-        if (present[v]) members.push_back(v);
+        const int32_t v = ctx->index_ids[j];
+        if (present[v]) m.members.push_back(v);
       }
-      std::sort(members.begin(), members.end());
-      for (int32_t v : members) stamp[v] = (int32_t)p;
-      outc.assign(members.size(), 0);
-      int32_t ncand = 0;
-      for (size_t i = 0; i < members.size(); i++) {
-        int32_t v = members[i];
+      std::sort(m.members.begin(), m.members.end());
+      for (int32_t v : m.members) stamp[v] = (int32_t)p;
+    });
+    on_threads(hi - lo + 1, [&](int64_t i) {
+      const int64_t p = hi - i;
+      PartMembers& m = pm[i];
+      m.outc.assign(m.members.size(), 0);
+      for (size_t k = 0; k < m.members.size(); k++) {
+        const int32_t v = m.members[k];
         int32_t c = 0;
         for (int64_t e = ctx->adj_off[v]; e < ctx->adj_off[v + 1]; e++) {
-          int32_t u = ctx->adj_nbr[e];
+          const int32_t u = ctx->adj_nbr[e];
           if (present[u] && stamp[u] != (int32_t)p) c++;
         }
-        outc[i] = c;
-        if (c == 0) ncand++;
+        m.outc[k] = c;
+        if (c == 0) m.ncand++;
       }
+    });
+    for (int64_t p = hi; p >= lo; p--) {
+      const std::vector<int32_t>& members = pm[hi - p].members;
+      const std::vector<int32_t>& outc = pm[hi - p].outc;
+      const int32_t ncand = pm[hi - p].ncand;
       ctx->nan_start[p] = (int32_t)ctx->elim.size();
       ctx->nan_max[p] = (int32_t)members.size();
       ctx->nan_count[p] = ncand;
