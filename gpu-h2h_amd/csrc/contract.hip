@@ -404,6 +404,21 @@ __global__ void __launch_bounds__(1024) extract_bags_kernel(BagArgs a) {
   }
 }
 
+// The input adjacency into the contraction's rows (row v at cur_off[v], its
+// capacity beyond the arcs left as it is: only the first cur_len entries of
+// a row are read).  One thread per vertex.
+__global__ void scatter_rows_kernel(const int64_t* __restrict__ adj_off, const int32_t* __restrict__ adj_nbr,
+                                    const int32_t* __restrict__ adj_w, const int64_t* __restrict__ cur_off,
+                                    int32_t* __restrict__ cur_nbr, int32_t* __restrict__ cur_w, int n) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t a0 = adj_off[v], a1 = adj_off[v + 1], o = cur_off[v];
+  for (int64_t e = a0; e < a1; e++) {
+    cur_nbr[o + (e - a0)] = adj_nbr[e];
+    cur_w[o + (e - a0)] = adj_w[e];
+  }
+}
+
 template <class T>
 h2h_status up(h2h_ctx* ctx, DevArray& d, const std::vector<T>& h) {
   return upload(ctx, d, h.data(), h.size() * sizeof(T));
@@ -438,18 +453,27 @@ h2h_status run_contraction(h2h_ctx* ctx, double* usec) {
     cur_len[v] = orig_deg[v];
     tot += std::max(cap[v], orig_deg[v]);
   }
-  std::vector<int32_t> cur_nbr(std::max<int64_t>(tot, 1)), cur_w(std::max<int64_t>(tot, 1));
-  for (int v = 0; v < n; v++) {
-    std::copy(ctx->adj_nbr.begin() + ctx->adj_off[v], ctx->adj_nbr.begin() + ctx->adj_off[v + 1], cur_nbr.begin() + cur_off[v]);
-    std::copy(ctx->adj_w.begin() + ctx->adj_off[v], ctx->adj_w.begin() + ctx->adj_off[v + 1], cur_w.begin() + cur_off[v]);
-  }
   DevArray d_cur_off, d_cur_len, d_cur_cap, d_cur_nbr, d_cur_w, d_orig, d_height;
   DevArray d_elim, d_rdeg, d_rhgt, d_epos, d_blen, d_err, d_W;
   H2H_TRY(up(ctx, d_cur_off, cur_off));
   H2H_TRY(up(ctx, d_cur_len, cur_len));
   H2H_TRY(up(ctx, d_cur_cap, cap));
-  H2H_TRY(up(ctx, d_cur_nbr, cur_nbr));
-  H2H_TRY(up(ctx, d_cur_w, cur_w));
+  {
+    // the compact adjacency goes up; the rows are laid out on the device
+    DevArray d_aoff, d_anbr, d_aw;
+    H2H_TRY(up(ctx, d_aoff, ctx->adj_off));
+    const size_t na = std::max<size_t>(ctx->adj_nbr.size(), 1);
+    const int32_t pad[1] = {0};  // a graph without arcs still gets valid buffers
+    H2H_TRY(upload(ctx, d_anbr, ctx->adj_nbr.empty() ? pad : ctx->adj_nbr.data(), na * sizeof(int32_t)));
+    H2H_TRY(upload(ctx, d_aw, ctx->adj_w.empty() ? pad : ctx->adj_w.data(), na * sizeof(int32_t)));
+    H2H_TRY(dev_alloc(ctx, d_cur_nbr, std::max<int64_t>(tot, 1) * sizeof(int32_t)));
+    H2H_TRY(dev_alloc(ctx, d_cur_w, std::max<int64_t>(tot, 1) * sizeof(int32_t)));
+    hipLaunchKernelGGL(scatter_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_aoff.as<int64_t>(),
+                       d_anbr.as<int32_t>(), d_aw.as<int32_t>(), d_cur_off.as<int64_t>(), d_cur_nbr.as<int32_t>(),
+                       d_cur_w.as<int32_t>(), n);
+    H2H_HIP(ctx, hipGetLastError());
+    H2H_HIP(ctx, hipStreamSynchronize(s));  // before the compact arrays are freed
+  }
   H2H_TRY(up(ctx, d_orig, orig_deg));
   H2H_TRY(dev_alloc(ctx, d_height, n * sizeof(int32_t)));
   H2H_HIP(ctx, hipMemsetAsync(d_height.p, 0, n * sizeof(int32_t), s));
