@@ -1608,7 +1608,6 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
                 o.cl_rc.push_back(ctx->dis_hash[ctx->pos_id[e]]);
                 o.cl_w.push_back(ctx->pos_w[e]);
               }
-            for (int c2 = 0; c2 <= d; c2++) o.cl_dst.push_back(-1);  // (no closure pass)
           } else {
             int x = v;
             anc_local.resize(d + 1);
@@ -1773,18 +1772,29 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     P.tlev_a.clear();
     if (lform) {
       // apply units by task depth across all parts (a part's units carry
-      // part-local node and entry indices, shifted here)
+      // part-local node and entry indices, shifted here): each (depth, part)
+      // slice's offset, then the copies on the threads
       size_t levels = 0;
       for (const Part& o : parts) levels = std::max(levels, o.lunits.size());
-      aunits.clear();
+      std::vector<size_t> at((levels + 1) * T + 1, 0);
+      size_t tot = 0;
       for (size_t dl = 0; dl < levels; dl++) {
-        P.tlev_a.push_back((int)aunits.size());
-        for (int i = 0; i < T; i++)
-          if (dl < parts[i].lunits.size())
-            for (const int4& uu : parts[i].lunits[dl])
-              aunits.push_back(make_int4(uu.x + (int)base[i].nb, uu.y, uu.z, uu.y < 0 ? uu.w : uu.w + (int)base[i].cl));
+        P.tlev_a.push_back((int)tot);
+        for (int i = 0; i < T; i++) {
+          at[dl * T + i] = tot;
+          tot += dl < parts[i].lunits.size() ? parts[i].lunits[dl].size() : 0;
+        }
       }
-      P.tlev_a.push_back((int)aunits.size());
+      P.tlev_a.push_back((int)tot);
+      aunits.assign(tot, make_int4(0, 0, 0, 0));
+      on_threads([&](int i) {
+        const int nb = (int)base[i].nb, cl = (int)base[i].cl;
+        for (size_t dl = 0; dl < parts[i].lunits.size(); dl++) {
+          size_t k = at[dl * T + i];
+          for (const int4& uu : parts[i].lunits[dl])
+            aunits[k++] = make_int4(uu.x + nb, uu.y, uu.z, uu.y < 0 ? uu.w : uu.w + cl);
+        }
+      });
     }
     host_lap("  plan: task lists");
     P.tbound_n = (int)bunits.size() - P.tbound_b;
@@ -1805,8 +1815,10 @@ h2h_status plan_batched(h2h_ctx* ctx, LabelPlan& P) {
     H2H_TRY(upload(ctx, P.task_bn_ie, bn_ie.data(), bn_ie.size() * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_ie_s, ie_s.data(), ie_s.size()));
     H2H_TRY(upload(ctx, P.task_ie_w, ie_w.data(), ie_w.size() * sizeof(int32_t)));
-    H2H_TRY(upload(ctx, P.task_cl_ro, cl_rc.data(), std::max<size_t>(cl_rc.size(), 1) * sizeof(int64_t)));
-    H2H_TRY(upload(ctx, P.task_cl_dst, cl_dst.data(), std::max<size_t>(cl_dst.size(), 1) * sizeof(int32_t)));
+    if (cl_rc.empty()) cl_rc.push_back(0);    // (valid buffers for the kernels' arguments)
+    if (cl_dst.empty()) cl_dst.push_back(-1);  // (level form: no closure pass reads it)
+    H2H_TRY(upload(ctx, P.task_cl_ro, cl_rc.data(), cl_rc.size() * sizeof(int64_t)));
+    H2H_TRY(upload(ctx, P.task_cl_dst, cl_dst.data(), cl_dst.size() * sizeof(int32_t)));
     if (lform && !cl_wh.empty()) H2H_TRY(upload(ctx, P.task_cl_w, cl_wh.data(), cl_wh.size() * sizeof(int32_t)));
     else H2H_TRY(dev_alloc(ctx, P.task_cl_w, std::max<size_t>(cl_rc.size(), 1) * sizeof(int32_t)));
     H2H_TRY(upload(ctx, P.task_units, aunits.data(), aunits.size() * sizeof(int4)));
