@@ -277,6 +277,11 @@ class H2HIndex:
         return out, us.value
 
     def query_device(self, x_ptr, y_ptr, out_ptr, n, kind=QUERY_LCA, exec_mode=EXEC_BATCHED, stream=None):
+        """Answer n queries held in device memory; ids outside [0, n) answer -1.
+        Enqueued on `stream` (a hipStream_t handle) without synchronising.  None
+        or 0 -- torch's default stream also has handle 0 -- means the index's own
+        stream, which is not ordered against the caller's: synchronise the device
+        before the call and before the output is read."""
         self._chk(lib().h2h_query_device(self.h, int(kind), int(exec_mode), ctypes.c_void_p(x_ptr),
                                          ctypes.c_void_p(y_ptr), ctypes.c_void_p(out_ptr), int(n),
                                          ctypes.c_void_p(stream) if stream else None))

@@ -1,0 +1,105 @@
+"""Query entry points at their edges, against the oracle: empty batches,
+s == t, vertex ids out of range (refused by the host-buffer entry point,
+answered -1 by the device-buffer one, the plain kernel's rule), and batch
+sizes around the batched kernel's tile of 256 queries."""
+import numpy as np
+import pytest
+import torch
+
+import cases
+import h2h
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+KINDS = (h2h.QUERY_LCA, h2h.QUERY_NOLCA)
+EXECS = (h2h.EXEC_PLAIN, h2h.EXEC_BATCHED)
+
+
+@pytest.fixture(scope="module")
+def index():
+    case = cases.grid64()
+    o = case.oracle(4)
+    o.contract(O.GPU_FAST, 1)
+    o.labels(O.SERIAL, l4=True)
+    p = h2h.H2HIndex()
+    case.product(p)
+    p.contract(1, h2h.CONTRACT_GPU_ORDER, 4)
+    p.build_labels(h2h.L4, h2h.BATCHED)
+    yield case, o, p
+    p.close()
+
+
+def _device_query(p, x, y, kind, ex):
+    xd = torch.from_numpy(np.ascontiguousarray(x, np.int32)).cuda()
+    yd = torch.from_numpy(np.ascontiguousarray(y, np.int32)).cuda()
+    out = torch.full((len(x),), 7, dtype=torch.int32, device="cuda")
+    # torch's default stream has handle 0, which the entry point reads as "the
+    # index's own stream": order the two streams by synchronising the device
+    torch.cuda.synchronize()
+    p.query_device(xd.data_ptr(), yd.data_ptr(), out.data_ptr(), len(x), kind, ex)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def test_empty_batch(index):
+    _, _, p = index
+    z = np.zeros(0, np.int32)
+    for kind in KINDS:
+        for ex in EXECS:
+            got, us = p.query(z, z, kind, ex)
+            assert got.shape == (0,) and us == 0
+            # n = 0 on the device entry point launches nothing and leaves the output alone
+            out = torch.full((4,), 7, dtype=torch.int32, device="cuda")
+            p.query_device(out.data_ptr(), out.data_ptr(), out.data_ptr(), 0, kind, ex)
+            torch.cuda.synchronize()
+            assert out.cpu().tolist() == [7, 7, 7, 7]
+
+
+def test_same_vertex_is_zero(index):
+    case, _, p = index
+    s = np.arange(0, case.n, 7, dtype=np.int32)
+    for kind in KINDS:
+        for ex in EXECS:
+            got, _ = p.query(s, s, kind, ex)
+            assert (got == 0).all()
+            assert (_device_query(p, s, s, kind, ex) == 0).all()
+
+
+def test_out_of_range_ids(index):
+    case, o, p = index
+    n = case.n
+    rng = np.random.default_rng(5)
+    x = rng.integers(0, n, 1000).astype(np.int32)
+    y = rng.integers(0, n, 1000).astype(np.int32)
+    bad = rng.choice(1000, 100, replace=False)
+    vals = np.array([-1, n, n + 1, 2**31 - 1, -2**31], np.int32)
+    x[bad[:50]] = vals[np.arange(50) % 5]
+    y[bad[50:]] = vals[np.arange(50) % 5]
+    ok = np.ones(1000, bool)
+    ok[bad] = False
+    for kind in KINDS:
+        ref, _ = o.query(kind, x[ok], y[ok], threads=4)
+        for ex in EXECS:
+            with pytest.raises(h2h.H2HError) as e:
+                p.query(x, y, kind, ex)
+            assert e.value.status == -1
+            got = _device_query(p, x, y, kind, ex)
+            assert (got[~ok] == -1).all()
+            np.testing.assert_array_equal(got[ok], ref)
+
+
+@pytest.mark.parametrize("nq", [1, 2, 63, 255, 256, 257, 511, 4097, 65537])
+def test_batch_sizes(index, nq):
+    """Ragged batches: tail tiles, single queries, and a mix of widths (each
+    tile groups its queries by LCA width) all answer as the oracle does."""
+    case, o, p = index
+    rng = np.random.default_rng(nq)
+    x = rng.integers(0, case.n, nq).astype(np.int32)
+    y = rng.integers(0, case.n, nq).astype(np.int32)
+    x[::11] = y[::11]
+    for kind in KINDS:
+        ref, _ = o.query(kind, x, y, threads=4)
+        for ex in EXECS:
+            got, _ = p.query(x, y, kind, ex)
+            np.testing.assert_array_equal(got, ref, err_msg=f"kind={kind} exec={ex}")
+            np.testing.assert_array_equal(_device_query(p, x, y, kind, ex), ref)
