@@ -159,6 +159,11 @@ h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
   // sizes fixed by the header must fit in what the file holds
   r.f.seekg(0, std::ios::end);
   const int64_t fsize = (int64_t)r.f.tellg();
+  // each count the file stores is at most its size in bytes: the products
+  // below cannot overflow (without labels, label_entries is only compared
+  // with the depths' sum)
+  if (h.n > fsize || h.nb > fsize || (h.has_labels && h.label_entries > fsize))
+    return set_err(ctx, H2H_ERR_IO, "index file size does not match its header");
   const int64_t need = (int64_t)sizeof(Header) + 8 * h.n + 8 * (h.n + 1) + 8 * h.nb + (h.has_dom ? h.nb : 0) +
                        (h.has_labels ? 4 * h.label_entries : 0) + 8;
   if (fsize != need) return set_err(ctx, H2H_ERR_IO, "index file size does not match its header");
@@ -189,8 +194,8 @@ h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
     else if (father[v] >= n || depth[father[v]] + 1 != depth[v]) return set_err(ctx, H2H_ERR_IO, "father/depth inconsistent");
     tot += depth[v];
     for (int64_t e = bag_off[v]; e < bag_off[v + 1]; e++)
-      if (bag_nbr[e] < 0 || bag_nbr[e] >= n || depth[bag_nbr[e]] >= depth[v])
-        return set_err(ctx, H2H_ERR_IO, "bag member out of range or not above its vertex");
+      if (bag_nbr[e] < 0 || bag_nbr[e] >= n || depth[bag_nbr[e]] >= depth[v] || bag_w[e] < 0)
+        return set_err(ctx, H2H_ERR_IO, "bag member out of range, not above its vertex, or of negative weight");
   }
   if (roots != 1) return set_err(ctx, H2H_ERR_IO, "the saved tree needs exactly one root of depth 1");
   if (tot != h.label_entries) return set_err(ctx, H2H_ERR_IO, "label rows do not match the depths");

@@ -75,8 +75,10 @@ def test_damaged_files_are_refused(tmp_path):
     data = bytearray(good.read_bytes())
     flipped = bytearray(data)
     flipped[len(data) // 2] ^= 0x40
+    huge = bytearray(data)
+    huge[24:32] = (1 << 61).to_bytes(8, "little")   # bag entries: 8 x this wraps a 64-bit size
     cases_bad = {"flip.h2h": bytes(flipped), "cut.h2h": bytes(data[:-100]), "magic.h2h": b"XXXX" + bytes(data[4:]),
-                 "empty.h2h": b""}
+                 "empty.h2h": b"", "huge.h2h": bytes(huge)}
     with h2h.H2HIndex() as q:
         for name, blob in cases_bad.items():
             (tmp_path / name).write_bytes(blob)
