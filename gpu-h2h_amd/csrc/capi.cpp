@@ -193,23 +193,31 @@ h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_fi
   if (!rid.open(id_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + id_file);
   if (!re.open(edge_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + edge_file);
   if (!rc.open(cut_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + cut_file);
+  // counts from the files are checked against what the file can hold (a
+  // number and a separator take two bytes at least) before anything is sized
+  // by them; values that do not fit int32 are malformed (the range checks of
+  // h2h_set_graph judge the rest)
+  auto fits32 = [](long long x) { return x >= INT32_MIN && x <= INT32_MAX; };
   long long n = 0;
   if (!rid.i64(n) || n < 1 || n > INT32_MAX) return set_err(ctx, H2H_ERR_IO, "ID file: bad vertex count");
+  if (n > (long long)rid.buf.size() / 8 + 1) return set_err(ctx, H2H_ERR_IO, "ID file: fewer lines than its vertex count");
   std::vector<int32_t> ids(n, -1);
   for (long long i = 0; i < n; i++) {
     long long idx, id;
-    if (!rid.i64(idx) || !rid.any() || !rid.any() || !rid.i64(id) || idx < 0 || idx >= n)
+    if (!rid.i64(idx) || !rid.any() || !rid.any() || !rid.i64(id) || idx < 0 || idx >= n || !fits32(id))
       return set_err(ctx, H2H_ERR_IO, "ID file: malformed line");
     ids[idx] = (int32_t)id;
   }
   long long en = 0, m = 0;
   if (!re.i64(en) || !re.i64(m) || m < 0) return set_err(ctx, H2H_ERR_IO, "Edge file: bad header");
   if (en != n) return set_err(ctx, H2H_ERR_IO, "Edge and ID files disagree on the vertex count");
+  if (m > (long long)re.buf.size() / 8 + 1) return set_err(ctx, H2H_ERR_IO, "Edge file: fewer lines than its arc count");
   std::vector<int32_t> u(m), v(m), w(m);
   std::string tag;
   for (long long i = 0; i < m; i++) {
     long long a, b, c;
     if (!re.token(tag) || !re.i64(a) || !re.i64(b) || !re.i64(c)) return set_err(ctx, H2H_ERR_IO, "Edge file: truncated");
+    if (!fits32(a - 1) || !fits32(b - 1) || !fits32(c)) return set_err(ctx, H2H_ERR_IO, "Edge file: value out of range");
     u[i] = (int32_t)(a - 1);
     v[i] = (int32_t)(b - 1);
     w[i] = (int32_t)c;
@@ -219,10 +227,12 @@ h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_fi
   const int H = std::min<int>((int)hf, tree_height);
   if (H < 1) return set_err(ctx, H2H_ERR_ARG, "partition height must be >= 1");
   const int64_t heapN = (1LL << H) - 1;
+  if (heapN > (int64_t)rc.buf.size() / 6 + 1) return set_err(ctx, H2H_ERR_IO, "Cut file: truncated");
   std::vector<int32_t> lr(2 * heapN);
   for (int64_t i = 0; i < heapN; i++) {
     long long h, l, r;
     if (!rc.i64(h) || !rc.i64(l) || !rc.i64(r)) return set_err(ctx, H2H_ERR_IO, "Cut file: truncated");
+    if (!fits32(l) || !fits32(r)) return set_err(ctx, H2H_ERR_IO, "Cut file: value out of range");
     lr[2 * i] = (int32_t)l;
     lr[2 * i + 1] = (int32_t)r;
   }

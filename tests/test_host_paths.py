@@ -183,6 +183,20 @@ def test_file_errors(tmp_path):
         with pytest.raises(h2h.H2HError) as e:
             p.load_files(str(bad), idf, cut, 4)
         assert e.value.status == -2
+        # counts beyond what the file can hold are refused before anything is
+        # sized by them; values that do not fit int32 are malformed
+        ref_edge = open(edge).read().split("\n", 1)[1]
+        for name, text, files in (
+                ("count", "13 1000000000000\na 1 2 4\n", lambda f: (f, idf, cut)),
+                ("wrap", "13 2\na 4294967297 2 4\na 2 1 4\n", lambda f: (f, idf, cut)),
+                ("weight", "13 44\n" + ref_edge.replace(" 2 4\n", " 2 4294967300\n", 1), lambda f: (f, idf, cut)),
+                ("ids", "1000000000\n0 1.0 6.0 2\n", lambda f: (edge, f, cut)),
+                ("cut", "30\n0 0 13\n", lambda f: (edge, idf, f))):
+            f = tmp_path / f"{name}.txt"
+            f.write_text(text)
+            with pytest.raises(h2h.H2HError) as e:
+                p.load_files(*(str(x) for x in files(f)), 30)
+            assert e.value.status == -2, name
         p.load_files(edge, idf, cut, 9)          # height argument above the file's: min(4, 9)
         assert p.info()["partition_height"] == 4
         p.load_files(edge, idf, cut, 2)
