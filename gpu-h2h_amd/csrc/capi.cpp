@@ -103,7 +103,7 @@ void h2h_destroy(h2h_ctx* ctx) {
 
 const char* h2h_last_error(const h2h_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-h2h_status h2h_set_graph(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_u, const int32_t* arc_v,
+static h2h_status set_graph_body(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_u, const int32_t* arc_v,
                          const int32_t* arc_w, const int32_t* index_ids, int height_file, int height_arg,
                          const int32_t* cut_lr) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
@@ -186,7 +186,15 @@ h2h_status h2h_set_graph(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_
   return H2H_OK;
 }
 
-h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_file, const char* cut_file,
+h2h_status h2h_set_graph(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_u, const int32_t* arc_v,
+                         const int32_t* arc_w, const int32_t* index_ids, int height_file, int height_arg,
+                         const int32_t* cut_lr) {
+  return guarded(ctx, [&] {
+    return set_graph_body(ctx, n, m, arc_u, arc_v, arc_w, index_ids, height_file, height_arg, cut_lr);
+  });
+}
+
+static h2h_status load_files_body(h2h_ctx* ctx, const char* edge_file, const char* id_file, const char* cut_file,
                           int tree_height) {
   if (need_ctx(ctx) || !edge_file || !id_file || !cut_file) return H2H_ERR_ARG;
   TextReader rid, re, rc;
@@ -239,7 +247,12 @@ h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_fi
   return h2h_set_graph(ctx, (int32_t)n, m, u.data(), v.data(), w.data(), ids.data(), (int)hf, tree_height, lr.data());
 }
 
-h2h_status h2h_partition_rank_tree(h2h_ctx* ctx, int32_t* nan_hash, int32_t* elim, double* layer_fraction) {
+h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_file, const char* cut_file,
+                          int tree_height) {
+  return guarded(ctx, [&] { return load_files_body(ctx, edge_file, id_file, cut_file, tree_height); });
+}
+
+static h2h_status partition_rank_tree_body(h2h_ctx* ctx, int32_t* nan_hash, int32_t* elim, double* layer_fraction) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (!ctx->graph_loaded) return set_err(ctx, H2H_ERR_STATE, "no graph loaded");
   h2h_status st = build_partition_plans(ctx, 1);
@@ -258,7 +271,11 @@ h2h_status h2h_partition_rank_tree(h2h_ctx* ctx, int32_t* nan_hash, int32_t* eli
   return H2H_OK;
 }
 
-h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int thread_num, double* usec) {
+h2h_status h2h_partition_rank_tree(h2h_ctx* ctx, int32_t* nan_hash, int32_t* elim, double* layer_fraction) {
+  return guarded(ctx, [&] { return partition_rank_tree_body(ctx, nan_hash, elim, layer_fraction); });
+}
+
+static h2h_status contract_body(h2h_ctx* ctx, int change_height, int contract_order, int thread_num, double* usec) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;
   if (!ctx->graph_loaded) return set_err(ctx, H2H_ERR_STATE, "no graph loaded");
@@ -291,7 +308,11 @@ h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int
   return H2H_OK;
 }
 
-h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* usec) {
+h2h_status h2h_contract(h2h_ctx* ctx, int change_height, int contract_order, int thread_num, double* usec) {
+  return guarded(ctx, [&] { return contract_body(ctx, change_height, contract_order, thread_num, usec); });
+}
+
+static h2h_status build_labels_body(h2h_ctx* ctx, int label_kind, int method, double* usec) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;
   if (!ctx->tree_ready) return set_err(ctx, H2H_ERR_STATE, "contract (or upload an index) first");
@@ -303,7 +324,11 @@ h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* us
   return run_label_build(ctx, label_kind, method, usec);
 }
 
-h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_result* best, h2h_tune_result* tried,
+h2h_status h2h_build_labels(h2h_ctx* ctx, int label_kind, int method, double* usec) {
+  return guarded(ctx, [&] { return build_labels_body(ctx, label_kind, method, usec); });
+}
+
+static h2h_status tune_batched_body(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_result* best, h2h_tune_result* tried,
                             int max_tried, int* ntried) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;
@@ -392,6 +417,11 @@ h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_
   return H2H_OK;
 }
 
+h2h_status h2h_tune_batched(h2h_ctx* ctx, int label_kind, int replays, h2h_tune_result* best, h2h_tune_result* tried,
+                            int max_tried, int* ntried) {
+  return guarded(ctx, [&] { return tune_batched_body(ctx, label_kind, replays, best, tried, max_tried, ntried); });
+}
+
 h2h_status h2h_get_batched_config(h2h_ctx* ctx, h2h_tune_result* cfg) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (!cfg) return set_err(ctx, H2H_ERR_ARG, "null config");
@@ -433,7 +463,7 @@ h2h_status h2h_set_batched_config(h2h_ctx* ctx, const h2h_tune_result* cfg) {
   return H2H_OK;
 }
 
-h2h_status h2h_build_labels_profiled(h2h_ctx* ctx, int label_kind, int method, double* kernel_us, int64_t* launches) {
+static h2h_status build_labels_profiled_body(h2h_ctx* ctx, int label_kind, int method, double* kernel_us, int64_t* launches) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;
   if (!ctx->tree_ready) return set_err(ctx, H2H_ERR_STATE, "contract (or upload an index) first");
@@ -445,7 +475,11 @@ h2h_status h2h_build_labels_profiled(h2h_ctx* ctx, int label_kind, int method, d
   return profile_label_build(ctx, label_kind, method, kernel_us, launches);
 }
 
-h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, const int32_t* dis,
+h2h_status h2h_build_labels_profiled(h2h_ctx* ctx, int label_kind, int method, double* kernel_us, int64_t* launches) {
+  return guarded(ctx, [&] { return build_labels_profiled_body(ctx, label_kind, method, kernel_us, launches); });
+}
+
+static h2h_status labels_upload_body(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, const int32_t* dis,
                              const int64_t* pos_hash, const int32_t* pos_id, const int32_t* pos_pos,
                              const int32_t* father, const int32_t* depth) {
   if (need_ctx(ctx) || n < 1 || !dis_hash || !dis || !pos_hash || !pos_id || !pos_pos || !father || !depth)
@@ -507,6 +541,12 @@ h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, c
   return upload_index(ctx);
 }
 
+h2h_status h2h_labels_upload(h2h_ctx* ctx, int32_t n, const int64_t* dis_hash, const int32_t* dis,
+                             const int64_t* pos_hash, const int32_t* pos_id, const int32_t* pos_pos,
+                             const int32_t* father, const int32_t* depth) {
+  return guarded(ctx, [&] { return labels_upload_body(ctx, n, dis_hash, dis, pos_hash, pos_id, pos_pos, father, depth); });
+}
+
 h2h_status h2h_query_device(h2h_ctx* ctx, int kind, int exec, const int32_t* d_x, const int32_t* d_y,
                             int32_t* d_out, int64_t n, void* stream) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
@@ -520,7 +560,7 @@ h2h_status h2h_query_device(h2h_ctx* ctx, int kind, int exec, const int32_t* d_x
   return launch_query(ctx, kind, exec, d_x, d_y, d_out, n, s);
 }
 
-h2h_status h2h_query(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const int32_t* y, int32_t* out,
+static h2h_status query_body(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const int32_t* y, int32_t* out,
                      int64_t n, double* usec) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;
@@ -563,6 +603,11 @@ h2h_status h2h_query(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const i
   (void)hipEventDestroy(e1);
   if (usec) *usec = ms * 1000.0;
   return H2H_OK;
+}
+
+h2h_status h2h_query(h2h_ctx* ctx, int kind, int exec, const int32_t* x, const int32_t* y, int32_t* out,
+                     int64_t n, double* usec) {
+  return guarded(ctx, [&] { return query_body(ctx, kind, exec, x, y, out, n, usec); });
 }
 
 h2h_status h2h_get_info(const h2h_ctx* ctx, h2h_info* o) {

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <deque>
+#include <exception>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -226,6 +227,29 @@ namespace h2h {
 // error helpers: record a message on the context and return the status
 h2h_status set_err(h2h_ctx* ctx, h2h_status st, const std::string& msg);
 h2h_status hip_err(h2h_ctx* ctx, hipError_t e, const char* what);
+
+// Entry points that size host memory from their inputs run their body here:
+// a C++ exception (a refused allocation, a thread that cannot start) becomes
+// H2H_ERR_OOM with a message instead of crossing the C boundary.
+template <class F>
+h2h_status guarded(h2h_ctx* ctx, F&& body) noexcept {
+  const char* what = "host allocation failed";
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+  } catch (const std::exception& e) {
+    what = e.what();
+  } catch (...) {
+    what = "unknown C++ exception";
+  }
+  if (ctx) {
+    try {
+      set_err(ctx, H2H_ERR_OOM, std::string("host resources: ") + what);
+    } catch (...) {
+    }
+  }
+  return H2H_ERR_OOM;
+}
 
 #define H2H_HIP(ctx, call)                                    \
   do {                                                        \

@@ -152,7 +152,7 @@ h2h_status host_walks(h2h_ctx* ctx, int kind, const int32_t* s, const int32_t* t
 
 extern "C" {
 
-h2h_status h2h_shortest_paths(h2h_ctx* ctx, const int32_t* s, const int32_t* t, int64_t npairs,
+static h2h_status shortest_paths_body(h2h_ctx* ctx, const int32_t* s, const int32_t* t, int64_t npairs,
                               int64_t* total_vertices, double* usec) {
   if (!ctx) return H2H_ERR_ARG;
   if (ctx->device < 0) return set_err(ctx, H2H_ERR_NOGPU, "host-only context (created with device -1)");
@@ -224,6 +224,11 @@ h2h_status h2h_shortest_paths(h2h_ctx* ctx, const int32_t* s, const int32_t* t, 
   if (total_vertices) *total_vertices = ctx->path_off[npairs];
   if (usec) *usec = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
   return H2H_OK;
+}
+
+h2h_status h2h_shortest_paths(h2h_ctx* ctx, const int32_t* s, const int32_t* t, int64_t npairs,
+                              int64_t* total_vertices, double* usec) {
+  return guarded(ctx, [&] { return shortest_paths_body(ctx, s, t, npairs, total_vertices, usec); });
 }
 
 h2h_status h2h_get_paths(h2h_ctx* ctx, int64_t* path_off, int32_t* vertices) {

@@ -103,7 +103,7 @@ struct Reader {
 
 extern "C" {
 
-h2h_status h2h_index_save(h2h_ctx* ctx, const char* path, int with_labels) {
+static h2h_status index_save_body(h2h_ctx* ctx, const char* path, int with_labels) {
   if (!ctx || !path) return H2H_ERR_ARG;
   if (!ctx->labels_ready) return set_err(ctx, H2H_ERR_STATE, "build (or load) labels before saving the index");
   const int64_t n = ctx->n, nb = ctx->bag_off[n], L = ctx->label_entries;
@@ -141,7 +141,11 @@ h2h_status h2h_index_save(h2h_ctx* ctx, const char* path, int with_labels) {
   return H2H_OK;
 }
 
-h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
+h2h_status h2h_index_save(h2h_ctx* ctx, const char* path, int with_labels) {
+  return guarded(ctx, [&] { return index_save_body(ctx, path, with_labels); });
+}
+
+static h2h_status index_load_body(h2h_ctx* ctx, const char* path) {
   if (!ctx || !path) return H2H_ERR_ARG;
   if (ctx->device < 0) return set_err(ctx, H2H_ERR_NOGPU, "host-only context (created with device -1)");
   if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the index");
@@ -243,6 +247,10 @@ h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
   ctx->label_kind = h.label_kind;
   ctx->labels_ready = true;
   return H2H_OK;
+}
+
+h2h_status h2h_index_load(h2h_ctx* ctx, const char* path) {
+  return guarded(ctx, [&] { return index_load_body(ctx, path); });
 }
 
 }  // extern "C"

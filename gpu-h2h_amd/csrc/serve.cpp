@@ -186,7 +186,7 @@ using namespace h2h;
 
 extern "C" {
 
-h2h_status h2h_server_start(h2h_ctx* ctx, const h2h_server_cfg* cfg) {
+static h2h_status server_start_body(h2h_ctx* ctx, const h2h_server_cfg* cfg) {
   if (!ctx || !cfg) return H2H_ERR_ARG;
   if (ctx->device < 0) return set_err(ctx, H2H_ERR_NOGPU, "host-only context");
   if (!ctx->labels_ready) return set_err(ctx, H2H_ERR_STATE, "build labels before serving");
@@ -219,7 +219,11 @@ h2h_status h2h_server_start(h2h_ctx* ctx, const h2h_server_cfg* cfg) {
   return H2H_OK;
 }
 
-h2h_status h2h_submit(h2h_ctx* ctx, const int32_t* x, const int32_t* y, int32_t* out, int64_t n, uint64_t* ticket) {
+h2h_status h2h_server_start(h2h_ctx* ctx, const h2h_server_cfg* cfg) {
+  return guarded(ctx, [&] { return server_start_body(ctx, cfg); });
+}
+
+static h2h_status submit_body(h2h_ctx* ctx, const int32_t* x, const int32_t* y, int32_t* out, int64_t n, uint64_t* ticket) {
   if (!ctx || !ticket || n < 1 || !x || !y || !out) return H2H_ERR_ARG;
   Server* S = ctx->server.get();
   if (!S) return H2H_ERR_STATE;
@@ -242,6 +246,10 @@ h2h_status h2h_submit(h2h_ctx* ctx, const int32_t* x, const int32_t* y, int32_t*
   }
   S->cv_sched.notify_one();
   return H2H_OK;
+}
+
+h2h_status h2h_submit(h2h_ctx* ctx, const int32_t* x, const int32_t* y, int32_t* out, int64_t n, uint64_t* ticket) {
+  return guarded(ctx, [&] { return submit_body(ctx, x, y, out, n, ticket); });
 }
 
 h2h_status h2h_wait(h2h_ctx* ctx, uint64_t ticket, int timeout_ms, h2h_req_stats* stats) {
