@@ -204,7 +204,7 @@ static h2h_status load_files_body(h2h_ctx* ctx, const char* edge_file, const cha
   // counts from the files are checked against what the file can hold (a
   // number and a separator take two bytes at least) before anything is sized
   // by them; values that do not fit int32 are malformed (the range checks of
-  // h2h_set_graph judge the rest)
+  // h2h_set_graph check the rest)
   auto fits32 = [](long long x) { return x >= INT32_MIN && x <= INT32_MAX; };
   long long n = 0;
   if (!rid.i64(n) || n < 1 || n > INT32_MAX) return set_err(ctx, H2H_ERR_IO, "ID file: bad vertex count");
