@@ -158,6 +158,27 @@ def test_uneven_partitions(order):
         _compare_queries(o, p, x, y)
 
 
+def test_contract_again_on_one_context():
+    """One context contracted three times (other C, other order): each time the
+    same record, tree, labels and answers as a fresh oracle; device buffers
+    grow or are reused and the captured label graphs are planned again."""
+    case = cases.delaunay(30000, 2.6, 5, 7)
+    rng = np.random.default_rng(17)
+    x = rng.integers(0, case.n, 20000).astype(np.int32)
+    y = rng.integers(0, case.n, 20000).astype(np.int32)
+    with h2h.H2HIndex() as p:
+        case.product(p)
+        for C, order in ((1, h2h.CONTRACT_GPU_ORDER), (case.H, h2h.CONTRACT_CPU_ORDER), (3, h2h.CONTRACT_GPU_ORDER)):
+            print(f"contract C={C} order={order}", flush=True)
+            o = case.oracle(4)
+            o.contract(O.CPU if order == h2h.CONTRACT_CPU_ORDER else O.GPU_FAST, C)
+            p.contract(C, order, 4)
+            _compare_structure(o, p, exact_record=True)
+            for m in (h2h.BATCHED, h2h.PF):
+                _compare_labels(o, p, h2h.L4, m)
+            _compare_queries(o, p, x, y)
+
+
 def test_labels_upload_roundtrip():
     """Bring-your-own index: the rows mallocH2HLabel_noHub leaves (bag weights,
     0 on the diagonal) plus pos/father/depth, built on the GPU."""
