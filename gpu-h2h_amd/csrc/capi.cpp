@@ -109,6 +109,7 @@ static h2h_status set_graph_body(h2h_ctx* ctx, int32_t n, int64_t m, const int32
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (n < 1 || m < 0 || (m > 0 && (!arc_u || !arc_v || !arc_w)) || !index_ids || !cut_lr)
     return set_err(ctx, H2H_ERR_ARG, "h2h_set_graph: bad sizes or null arrays");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the graph");
   const int H = std::min(height_file, height_arg);
   if (H < 1 || H > 30) return set_err(ctx, H2H_ERR_ARG, "partition height must be in [1, 30]");
   ctx->graph_loaded = ctx->contracted = ctx->tree_ready = ctx->labels_ready = false;
@@ -197,6 +198,7 @@ h2h_status h2h_set_graph(h2h_ctx* ctx, int32_t n, int64_t m, const int32_t* arc_
 static h2h_status load_files_body(h2h_ctx* ctx, const char* edge_file, const char* id_file, const char* cut_file,
                           int tree_height) {
   if (need_ctx(ctx) || !edge_file || !id_file || !cut_file) return H2H_ERR_ARG;
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the graph");
   TextReader rid, re, rc;
   if (!rid.open(id_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + id_file);
   if (!re.open(edge_file)) return set_err(ctx, H2H_ERR_IO, std::string("cannot open ") + edge_file);
@@ -255,6 +257,7 @@ h2h_status h2h_load_files(h2h_ctx* ctx, const char* edge_file, const char* id_fi
 static h2h_status partition_rank_tree_body(h2h_ctx* ctx, int32_t* nan_hash, int32_t* elim, double* layer_fraction) {
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (!ctx->graph_loaded) return set_err(ctx, H2H_ERR_STATE, "no graph loaded");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before replacing the index");
   h2h_status st = build_partition_plans(ctx, 1);
   ctx->plans.clear();
   if (st) return st;
@@ -279,6 +282,7 @@ static h2h_status contract_body(h2h_ctx* ctx, int change_height, int contract_or
   if (need_ctx(ctx)) return H2H_ERR_ARG;
   if (need_gpu(ctx)) return H2H_ERR_NOGPU;
   if (!ctx->graph_loaded) return set_err(ctx, H2H_ERR_STATE, "no graph loaded");
+  if (ctx->server) return set_err(ctx, H2H_ERR_STATE, "stop the server before contracting again");
   if (change_height < 1 || change_height > ctx->H)
     return set_err(ctx, H2H_ERR_ARG, "ChangeToGlobalHeight must be in [1, PartitionHeight]");
   if (contract_order != H2H_CONTRACT_CPU_ORDER && contract_order != H2H_CONTRACT_GPU_ORDER)

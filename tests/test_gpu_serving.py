@@ -55,6 +55,7 @@ def test_alone_equals_in_mix(index, exec_mode):
 
 def test_admission_and_cancel(index):
     case, p = index
+    ref, _ = p.query(np.arange(100, dtype=np.int32), np.zeros(100, np.int32))
     p.server_start(streams=2, max_inflight=4096, max_request=1024)
     try:
         x = np.zeros(2048, np.int32)
@@ -67,5 +68,14 @@ def test_admission_and_cancel(index):
         t, out = p.submit(np.arange(100, dtype=np.int32), np.arange(100, dtype=np.int32))
         p.wait(t)
         np.testing.assert_array_equal(out, 0)
+        # nothing may replace the device index under the running server
+        for call in (lambda: p.contract(1), lambda: p.build_labels(h2h.L4, h2h.BATCHED),
+                     lambda: case.product(p), lambda: p.partition_rank_tree()):
+            with pytest.raises(h2h.H2HError) as e:
+                call()
+            assert e.value.status == -7  # H2H_ERR_STATE
+        t, out = p.submit(np.arange(100, dtype=np.int32), np.zeros(100, np.int32))
+        p.wait(t)
+        np.testing.assert_array_equal(out, ref)
     finally:
         p.server_stop()
